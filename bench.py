@@ -1,0 +1,209 @@
+"""Benchmark of the w-stacking imaging hot path on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic samples already
+resident in HBM: bucket + grid -> per-plane FFT -> w-phase correction + plane sum
+(lofar_image_dev).  The headline workload is the FP64 drop-in-mode configuration:
+16M samples, 4096x4096 grid, 32 w-planes, support 7.  The same run reports the
+FP32C precision mode on 64M samples (same grid), the adjoint (degridding) rate,
+image errors, the dominant kernel's roofline fraction (HIP events on the stream
+the kernels run on) and the reference CPU path timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("Mvis/s end-to-end imaging, FP32-comp vs FP64 at stated image RMS err; adjoint Mvis/s")
+
+
+def per_kernel_bytes(n, nx, ny, nw, cell_bytes):
+    """Algorithmic HBM bytes per launch: B = 36 N + 4 C + 16 Nx Ny split by kernel."""
+    C = nw * nx * ny * cell_bytes
+    return {"grid_tiles": 36 * n + C, "fft_cols": 2 * C, "rows_phase": C + 16 * nx * ny,
+            "total": 36 * n + 4 * C + 16 * nx * ny}
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def profile_stages(fn, reps):
+    import torch
+    from lofar_amd import imager
+    imager.profile_read()
+    imager.profile_enable(True)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    imager.profile_enable(False)
+    return {k: (ms / c, c // reps) for k, (ms, c) in imager.profile_read().items()}
+
+
+def rel_err(a_re, a_im, b_re, b_im):
+    import torch
+    num = torch.sqrt(torch.sum((a_re - b_re) ** 2 + (a_im - b_im) ** 2))
+    den = torch.sqrt(torch.sum(b_re ** 2 + b_im ** 2))
+    mx = torch.max(torch.sqrt((a_re - b_re) ** 2 + (a_im - b_im) ** 2))
+    mref = torch.max(torch.sqrt(b_re ** 2 + b_im ** 2))
+    return float(num / den), float(mx / mref)
+
+
+def cpu_baseline(n, nx, nw):
+    env = dict(os.environ)
+    env.setdefault("MKL_NUM_THREADS", "1")
+    env.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
+    try:
+        out = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"),
+                              "--n", str(n), "--nx", str(nx), "--nw", str(nw)],
+                             capture_output=True, text=True, env=env, timeout=300)
+        return json.loads(out.stdout.strip().splitlines()[-1])
+    except Exception as e:  # reported, never fatal
+        return {"error": repr(e)}
+
+
+def load_traffic(key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(key)
+    except ValueError:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the w-plane sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from lofar_amd import imager
+    from lofar_amd.synth import make_obs
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nx = ny = 4096
+    nw, support = 32, 7
+    n2 = 16 * 1024 * 1024
+    print("bench: generating %d samples (%dx%d, %d planes)" % (n2, nx, ny, nw), file=sys.stderr, flush=True)
+    d = make_obs(n2, nx, nw, seed=2024, backend="torch", device=dev)
+    vis = [d[k] for k in ("uu", "vv", "ww", "vis_real", "vis_img", "weight")]
+    cube64 = torch.empty((nw, ny, nx), dtype=torch.complex128, device=dev)
+    re64 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+    im64 = torch.empty_like(re64)
+
+    def step64():
+        imager.image(*vis, nw, support, nx, ny, d["resolution"], d["wmin"], d["wmax"],
+                     precision="fp64", cube=cube64, image_real=re64, image_imag=im64)
+
+    ms64 = time_steps(step64, a.steps, a.warmup)
+    value = n2 / (ms64 * 1e-3) / 1e6
+    print("bench: fp64 %.3f ms/step  %.1f Mvis/s" % (ms64, value), file=sys.stderr, flush=True)
+    prof = profile_stages(step64, 3)
+    kb = per_kernel_bytes(n2, nx, ny, nw, 16)
+    dom = max((k for k in prof if k in kb), key=lambda k: prof[k][0])
+    dom_ms = prof[dom][0] / max(prof[dom][1], 1)
+    achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": load_traffic("fp64_4096x4096x32_16M:" + dom),
+                "algorithmic_bytes": kb[dom], "avg_launch_ms": dom_ms,
+                "stages_ms": {k: round(v[0], 4) for k, v in prof.items()},
+                "end_to_end_GBs": kb["total"] / (ms64 * 1e-3) / 1e9}
+
+    extra = {}
+    if not a.no_extra:
+        # FP32C on 64M samples, same grid; error against the FP64 mode on the same input
+        del cube64
+        torch.cuda.empty_cache()
+        n3 = 64 * 1024 * 1024
+        d3 = make_obs(n3, nx, nw, seed=2025, backend="torch", device=dev)
+        vis3 = [d3[k] for k in ("uu", "vv", "ww", "vis_real", "vis_img", "weight")]
+        cube32 = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
+        re32 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+        im32 = torch.empty_like(re32)
+
+        def step32():
+            imager.image(*vis3, nw, support, nx, ny, d3["resolution"], d3["wmin"], d3["wmax"],
+                         precision="fp32c", cube=cube32, image_real=re32, image_imag=im32)
+
+        ms32 = time_steps(step32, a.steps, a.warmup)
+        prof32 = profile_stages(step32, 3)
+        del cube32
+        torch.cuda.empty_cache()
+        c64 = torch.empty((nw, ny, nx), dtype=torch.complex128, device=dev)
+        r64, i64 = imager.image(*vis3, nw, support, nx, ny, d3["resolution"], d3["wmin"],
+                                d3["wmax"], precision="fp64", cube=c64)
+        rms, mx = rel_err(re32, im32, r64, i64)
+        mvis32 = n3 / (ms32 * 1e-3) / 1e6
+        extra["fp32c"] = {"workload": "64M samples, 4096x4096, 32 w-planes, support 7, complex64 cube",
+                          "value": mvis32, "unit": "Mvis/s", "ms_per_step": ms32,
+                          "speedup_vs_fp64_mode": mvis32 / value,
+                          "image_rel_rms_err_vs_fp64": rms, "image_rel_max_err_vs_fp64": mx,
+                          "stages_ms": {k: round(v[0], 4) for k, v in prof32.items()}}
+        print("bench: fp32c %.3f ms/step %.1f Mvis/s err %.2e" % (ms32, mvis32, rms),
+              file=sys.stderr, flush=True)
+        del d3, vis3, c64
+        torch.cuda.empty_cache()
+
+        # adjoint: degridding a 4096^2 image to the 16M samples (FP32C)
+        cubeA = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
+        g = torch.Generator(device=dev)
+        g.manual_seed(5)
+        img_re = torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+        img_im = torch.zeros_like(img_re)
+        mr = torch.empty(n2, dtype=torch.float32, device=dev)
+        mi = torch.empty_like(mr)
+
+        def stepA():
+            imager.degrid(img_re, img_im, d["uu"], d["vv"], d["ww"], d["weight"], nw, support,
+                          d["resolution"], d["wmin"], d["wmax"], precision="fp32c", cube=cubeA,
+                          model_real=mr, model_imag=mi)
+
+        msA = time_steps(stepA, a.steps, a.warmup)
+        profA = profile_stages(stepA, 3)
+        extra["adjoint"] = {"workload": "degrid 4096x4096 image to 16M samples, 32 w-planes, FP32C",
+                            "value": n2 / (msA * 1e-3) / 1e6, "unit": "Mvis/s", "ms_per_step": msA,
+                            "stages_ms": {k: round(v[0], 4) for k, v in profA.items()}}
+        print("bench: adjoint %.3f ms/step" % msA, file=sys.stderr, flush=True)
+
+    cpu = None if a.no_cpu else cpu_baseline(n2, nx, nw)
+    line = {
+        "metric": METRIC, "value": value, "unit": "Mvis/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, lofar_amd.synth)",
+        "config": {"workload": "FP64 end-to-end imaging (grid+FFT+phase+plane sum), 16M samples, "
+                               "4096x4096 grid, 32 w-planes, support 7, inputs resident in HBM",
+                   "samples": n2, "grid": [nx, ny], "w_planes": nw, "support": support},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

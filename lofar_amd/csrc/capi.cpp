@@ -1,0 +1,725 @@
+// C-ABI layer of liblofar_hip: per-device context, workspace, twiddle tables,
+// the reference drop-ins (wstack, phase_correction, test) and the device entry
+// points.  Host pointers only cross PCIe in the drop-ins; the lofar_*_dev entry
+// points enqueue on the caller's stream and never synchronise except to read
+// back the sample-validation flag and the bucket total (16 bytes) per gridding.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+#include "fft_kernels.h"
+#include "grid_kernels.h"
+
+namespace lofar {
+namespace {
+
+thread_local int t_last_error = LOFAR_OK;
+
+struct Buffer {
+    void* ptr = nullptr;
+    size_t size = 0;
+};
+
+struct Prof {
+    int stage;
+    hipEvent_t a, b;
+};
+
+struct DeviceCtx {
+    bool ready = false;
+    int device = -1;
+    std::mutex mu;                  // serialises calls that use this device's workspace
+    hipStream_t stream = nullptr;   // stream of the host-pointer drop-ins
+    hipEvent_t last_done = nullptr; // end of the previous call's work
+    Buffer meta;                    // counts, offsets, scan partials
+    Buffer ents;                    // bucket entries and windows
+    Buffer host_io;                 // device copies of drop-in host arrays
+    long long* status_host = nullptr;  // pinned: [total, err]
+    std::map<long long, void*> twiddles;
+    std::vector<Prof> prof;
+    std::vector<hipEvent_t> pool;
+};
+
+DeviceCtx g_ctx[64];
+std::mutex g_init_mu;
+bool g_profiling = false;
+
+int fail(int code)
+{
+    t_last_error = code;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "liblofar_hip: %s failed: %s\n", #x, hipGetErrorString(e_));   \
+            return fail(e_ == hipErrorOutOfMemory ? LOFAR_ENOMEM : LOFAR_EHIP);             \
+        }                                                                                  \
+    } while (0)
+
+int get_ctx(DeviceCtx** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LOFAR_ENODEV;
+    if (dev < 0 || dev >= 64) return LOFAR_ENODEV;
+    DeviceCtx& c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (!c.ready) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return LOFAR_ENODEV;
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            fprintf(stderr, "liblofar_hip: device %d is %s, this build targets gfx950\n", dev,
+                    prop.gcnArchName);
+            return LOFAR_ENODEV;
+        }
+        if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess)
+            return LOFAR_EHIP;
+        if (hipEventCreateWithFlags(&c.last_done, hipEventDisableTiming) != hipSuccess)
+            return LOFAR_EHIP;
+        if (hipHostMalloc((void**)&c.status_host, 2 * sizeof(long long), hipHostMallocDefault) !=
+            hipSuccess)
+            return LOFAR_EHIP;
+        c.device = dev;
+        c.ready = true;
+    }
+    *out = &c;
+    return LOFAR_OK;
+}
+
+// grow-only device buffer; waits for the previous call before freeing
+int ensure(DeviceCtx& c, Buffer& b, size_t bytes)
+{
+    if (b.size >= bytes) return LOFAR_OK;
+    HIPCHK(hipEventSynchronize(c.last_done));
+    if (b.ptr) HIPCHK(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.size = 0;
+    size_t want = bytes + bytes / 8 + 4096;
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+        b.ptr = nullptr;
+        (void)hipGetLastError();
+        return fail(LOFAR_ENOMEM);
+    }
+    b.size = want;
+    return LOFAR_OK;
+}
+
+// exp(+2*pi*i*m/N), m < N, from long double arithmetic, rounded once
+int twiddles(DeviceCtx& c, int n, int precision, void** out)
+{
+    const long long key = (long long)n * 4 + precision;
+    auto it = c.twiddles.find(key);
+    if (it != c.twiddles.end()) {
+        *out = it->second;
+        return LOFAR_OK;
+    }
+    std::vector<double> tw(2 * (size_t)n);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int m = 0; m < n; m++) {
+        const long double a = two_pi * (long double)m / (long double)n;
+        tw[2 * m] = (double)cosl(a);
+        tw[2 * m + 1] = (double)sinl(a);
+    }
+    void* d = nullptr;
+    size_t bytes = (size_t)n * (precision == LOFAR_FP64 ? 16 : 8);
+    HIPCHK(hipMalloc(&d, bytes));
+    if (precision == LOFAR_FP64) {
+        HIPCHK(hipMemcpy(d, tw.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(2 * (size_t)n);
+        for (size_t i = 0; i < f.size(); i++) f[i] = (float)tw[i];
+        HIPCHK(hipMemcpy(d, f.data(), bytes, hipMemcpyHostToDevice));
+    }
+    c.twiddles[key] = d;
+    *out = d;
+    return LOFAR_OK;
+}
+
+hipEvent_t pool_event(DeviceCtx& c)
+{
+    if (!c.pool.empty()) {
+        hipEvent_t e = c.pool.back();
+        c.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs one stage's launches, bracketed by timing events when profiling is on.
+template <typename F>
+hipError_t stage(DeviceCtx& c, int id, hipStream_t st, F&& fn)
+{
+    if (!g_profiling) return fn();
+    Prof p{id, pool_event(c), pool_event(c)};
+    if (p.a) (void)hipEventRecord(p.a, st);
+    hipError_t e = fn();
+    if (p.b) (void)hipEventRecord(p.b, st);
+    if (p.a && p.b) c.prof.push_back(p);
+    return e;
+}
+
+enum Stage {
+    kStCount = 0, kStScan, kStScatter, kStGrid, kStCols, kStRows, kStRowsPhase, kStPhase,
+    kStRowsPhaseAdj, kStDegrid
+};
+const char* kStageNames[LOFAR_NSTAGES] = {
+    "bucket_count", "bucket_scan", "bucket_scatter", "grid_tiles", "fft_cols",
+    "fft_rows", "rows_phase", "phase_sum", "rows_phase_adj", "degrid_tiles"};
+
+bool pow2_ok(int n) { return n >= 2 && n <= 8192 && (n & (n - 1)) == 0; }
+
+int check_grid_args(int precision, int nw, long n, long F, long P, double dx, double dw,
+                    int support, int nx, int ny)
+{
+    if (precision != LOFAR_FP64 && precision != LOFAR_FP32C) return LOFAR_EARG;
+    if (nw < 1 || n < 0 || F < 1 || P < 1 || support < 1 || support > 17) return LOFAR_EARG;
+    if (nx < 1 || ny < 1 || nx > 65536 || ny > 65536) return LOFAR_EARG;
+    if (!(dx > 0.0) || !(dw > 0.0) || !std::isfinite(dx) || !std::isfinite(dw)) return LOFAR_EARG;
+    return LOFAR_OK;
+}
+
+Geometry make_geometry(int nw, int support, int nx, int ny, double dx, double dw)
+{
+    Geometry g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nw = nw;
+    g.K = (support - 1) / 2;
+    g.ntu = (nx + kTile - 1) / kTile;
+    g.ntv = (ny + kTile - 1) / kTile;
+    g.dx = dx;
+    g.dw = dw;
+    return g;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Counting sort of the samples into (w, tile) buckets; validates every sample
+// and returns the number of entries.  The only host synchronisation of a call.
+int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double* vv,
+                  const double* ww, const Geometry& g, hipStream_t st, Buckets& b,
+                  long long& total)
+{
+    b.nb = (long long)g.nw * g.ntv * g.ntu;
+    const long long nblk = (b.nb + 4095) / 4096;
+    const size_t off_status = 0;
+    const size_t off_counts = align256(off_status + 2 * sizeof(long long));
+    const size_t off_offsets = align256(off_counts + (size_t)b.nb * sizeof(unsigned));
+    const size_t off_partials = align256(off_offsets + (size_t)(b.nb + 1) * sizeof(long long));
+    const size_t bytes = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
+    int r = ensure(c, c.meta, bytes);
+    if (r) return r;
+    char* base = (char*)c.meta.ptr;
+    b.total = (long long*)(base + off_status);
+    b.err = (int*)(base + off_status + sizeof(long long));
+    b.counts = (unsigned*)(base + off_counts);
+    b.offsets = (long long*)(base + off_offsets);
+    b.partials = (long long*)(base + off_partials);
+    HIPCHK(stage(c, kStCount, st, [&] {
+        hipError_t e = bucket_zero(b, st);
+        if (e != hipSuccess) return e;
+        return bucket_count(mode, n, uu, vv, ww, g, b, st);
+    }));
+    HIPCHK(stage(c, kStScan, st, [&] { return bucket_scan(b, st); }));
+    HIPCHK(hipMemcpyAsync(c.status_host, b.total, 2 * sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int err = (int)(c.status_host[1] & 0xffffffff);
+    if (err) return fail(LOFAR_ECOORD);
+    total = c.status_host[0];
+    // counts become the insertion cursors of the scatter
+    HIPCHK(bucket_zero(b, st));
+    return LOFAR_OK;
+}
+
+int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long F, long P,
+              const double* uu, const double* vv, const double* ww, const float* vr,
+              const float* vi, const float* wt, double dx, double dw, int support, int nx, int ny,
+              void* cube, hipStream_t st)
+{
+    int r = check_grid_args(precision, nw, n, F, P, dx, dw, support, nx, ny);
+    if (r) return fail(r);
+    if (!cube || (n > 0 && (!uu || !vv || !ww || !vr || !vi || !wt))) return fail(LOFAR_EARG);
+    const Geometry g = make_geometry(nw, support, nx, ny, dx, dw);
+    Buckets b;
+    long long total = 0;
+    r = build_buckets(c, kBucketFootprint, n, uu, vv, ww, g, st, b, total);
+    if (r) return r;
+    const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
+    const size_t off_wins = align256((size_t)total * esz);
+    r = ensure(c, c.ents, off_wins + (size_t)total * sizeof(uint32_t) + 256);
+    if (r) return r;
+    void* ents = c.ents.ptr;
+    uint32_t* wins = (uint32_t*)((char*)c.ents.ptr + off_wins);
+    HIPCHK(stage(c, kStScatter, st, [&] {
+        return bucket_scatter_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, ents, wins,
+                                   st);
+    }));
+    HIPCHK(stage(c, kStGrid, st,
+                 [&] { return grid_tiles(precision, g, b, ents, wins, cube, accumulate, st); }));
+    return LOFAR_OK;
+}
+
+int fft_impl(DeviceCtx& c, int precision, int sign, const void* in, void* out, int nx, int ny,
+             int nw, double scale, hipStream_t st)
+{
+    if (precision != LOFAR_FP64 && precision != LOFAR_FP32C) return fail(LOFAR_EARG);
+    if (sign != 1 && sign != -1) return fail(LOFAR_EARG);
+    if (nw < 1 || !in || !out) return fail(LOFAR_EARG);
+    if (!pow2_ok(nx) || !pow2_ok(ny)) return fail(LOFAR_ESIZE);
+    void *twx, *twy;
+    int r = twiddles(c, nx, precision, &twx);
+    if (r) return r;
+    r = twiddles(c, ny, precision, &twy);
+    if (r) return r;
+    const long long lines = (long long)nw * ny;
+    if (precision == LOFAR_FP64) {
+        HIPCHK(stage(c, kStCols, st, [&] {
+            return fft_cols<double>(sign, in, out, twy, nx, ny, nw, 1.0, st);
+        }));
+        HIPCHK(stage(c, kStRows, st, [&] {
+            return fft_rows<double>(sign, out, out, twx, nx, lines, scale, st);
+        }));
+    } else {
+        HIPCHK(stage(c, kStCols, st, [&] {
+            return fft_cols<float>(sign, in, out, twy, nx, ny, nw, 1.0, st);
+        }));
+        HIPCHK(stage(c, kStRows, st, [&] {
+            return fft_rows<float>(sign, out, out, twx, nx, lines, scale, st);
+        }));
+    }
+    return LOFAR_OK;
+}
+
+int check_phase_args(int precision, int xaxis, int yaxis, int nw, int xtot, int ytot)
+{
+    if (precision != LOFAR_FP64 && precision != LOFAR_FP32C) return LOFAR_EARG;
+    if (xaxis < 1 || yaxis < 1 || nw < 1 || xtot < 1 || ytot < 1) return LOFAR_EARG;
+    return LOFAR_OK;
+}
+
+// Calls on one device are serialised on its mutex, and each call's work is
+// ordered after the previous call's (event wait), so the shared workspace is
+// never used by two calls at once even when they use different streams.
+struct CallGuard {
+    DeviceCtx* c = nullptr;
+    hipStream_t st = nullptr;
+    std::unique_lock<std::mutex> lk;
+    int status = LOFAR_OK;
+    explicit CallGuard(void* stream)
+    {
+        status = get_ctx(&c);
+        if (status) {
+            t_last_error = status;
+            return;
+        }
+        lk = std::unique_lock<std::mutex>(c->mu);
+        st = (hipStream_t)stream;
+        if (hipStreamWaitEvent(st, c->last_done, 0) != hipSuccess) status = fail(LOFAR_EHIP);
+    }
+    ~CallGuard()
+    {
+        if (c && lk.owns_lock()) (void)hipEventRecord(c->last_done, st);
+    }
+};
+
+void report_once(const char* what, int code)
+{
+    static bool printed = false;
+    if (!printed) {
+        fprintf(stderr, "liblofar_hip: %s failed: %s (further errors not printed)\n", what,
+                lofar_error_string(code));
+        printed = true;
+    }
+}
+
+}  // namespace
+}  // namespace lofar
+
+using namespace lofar;
+
+extern "C" {
+
+int lofar_last_error(void) { return t_last_error; }
+
+const char* lofar_error_string(int code)
+{
+    switch (code) {
+        case LOFAR_OK: return "ok";
+        case LOFAR_EARG: return "bad argument";
+        case LOFAR_ECOORD: return "sample not finite or outside the w-planes";
+        case LOFAR_ESIZE: return "FFT side must be a power of two in [2, 8192]";
+        case LOFAR_EHIP: return "HIP runtime error";
+        case LOFAR_ENOMEM: return "device allocation failed";
+        case LOFAR_ENODEV: return "no usable gfx950 device";
+        default: return "unknown error";
+    }
+}
+
+int lofar_version(void) { return 10000 * 0 + 100 * 1 + 0; }
+
+int test(int nnn) { return nnn + 1; }
+
+int lofar_profile_enable(int on)
+{
+    g_profiling = on != 0;
+    return LOFAR_OK;
+}
+
+int lofar_profile_read(double* total_ms, int* launches)
+{
+    DeviceCtx* c = nullptr;
+    int r = get_ctx(&c);
+    if (r) return fail(r);
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int i = 0; i < LOFAR_NSTAGES; i++) {
+        if (total_ms) total_ms[i] = 0.0;
+        if (launches) launches[i] = 0;
+    }
+    for (auto& p : c->prof) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            if (total_ms) total_ms[p.stage] += ms;
+            if (launches) launches[p.stage] += 1;
+        }
+        c->pool.push_back(p.a);
+        c->pool.push_back(p.b);
+    }
+    c->prof.clear();
+    return LOFAR_OK;
+}
+
+const char* lofar_stage_name(int stage)
+{
+    return (stage >= 0 && stage < LOFAR_NSTAGES) ? kStageNames[stage] : "unknown";
+}
+
+int lofar_grid_dev(int precision, int accumulate, int num_w_planes, long num_points,
+                   long freq_per_chan, long polarizations, const double* uu, const double* vv,
+                   const double* ww, const float* vis_real, const float* vis_img,
+                   const float* weight, double dx, double dw, int w_support, int grid_size_x,
+                   int grid_size_y, void* cube, void* stream)
+{
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    return grid_impl(*cg.c, precision, accumulate, num_w_planes, num_points, freq_per_chan,
+                     polarizations, uu, vv, ww, vis_real, vis_img, weight, dx, dw, w_support,
+                     grid_size_x, grid_size_y, cube, cg.st);
+}
+
+int lofar_fft_dev(int precision, int sign, const void* in, void* out, int nx, int ny, int nw,
+                  double scale, void* stream)
+{
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    return fft_impl(*cg.c, precision, sign, in, out, nx, ny, nw, scale, cg.st);
+}
+
+int lofar_phase_dev(int precision, const void* cube, double* image_real, double* image_imag,
+                    int xaxis, int yaxis, int num_w_planes, int xaxistot, int yaxistot,
+                    double resolution, double wmin, double wmax, int accumulate, void* stream)
+{
+    int r = check_phase_args(precision, xaxis, yaxis, num_w_planes, xaxistot, yaxistot);
+    if (r) return fail(r);
+    if (!cube || !image_real || !image_imag) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    HIPCHK(stage(c, kStPhase, cg.st, [&] {
+        return phase_sum(precision, cube, image_real, image_imag, xaxis, yaxis, num_w_planes,
+                         xaxistot, yaxistot, resolution, wmin, wmax, accumulate, cg.st);
+    }));
+    return LOFAR_OK;
+}
+
+int lofar_phase_adjoint_dev(int precision, const double* image_real, const double* image_imag,
+                            void* cube, int xaxis, int yaxis, int num_w_planes, int xaxistot,
+                            int yaxistot, double resolution, double wmin, double wmax, void* stream)
+{
+    int r = check_phase_args(precision, xaxis, yaxis, num_w_planes, xaxistot, yaxistot);
+    if (r) return fail(r);
+    if (!cube || !image_real || !image_imag) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    HIPCHK(stage(c, kStPhase, cg.st, [&] {
+        return phase_adjoint(precision, image_real, image_imag, cube, xaxis, yaxis, num_w_planes,
+                             xaxistot, yaxistot, resolution, wmin, wmax, cg.st);
+    }));
+    return LOFAR_OK;
+}
+
+int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,
+                    long polarizations, const double* uu, const double* vv, const double* ww,
+                    const float* vis_real, const float* vis_img, const float* weight, double dx,
+                    double dw, int w_support, int grid_size_x, int grid_size_y, double resolution,
+                    double wmin, double wmax, void* cube, double* image_real, double* image_imag,
+                    int accumulate, void* stream)
+{
+    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    if (!image_real || !image_imag) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    int r = grid_impl(c, precision, 0, num_w_planes, num_points, freq_per_chan, polarizations, uu,
+                      vv, ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x,
+                      grid_size_y, cube, cg.st);
+    if (r) return r;
+    void *twx, *twy;
+    r = twiddles(c, grid_size_x, precision, &twx);
+    if (r) return r;
+    r = twiddles(c, grid_size_y, precision, &twy);
+    if (r) return r;
+    const PhaseParams pp =
+        phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
+    const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
+    if (precision == LOFAR_FP64) {
+        HIPCHK(stage(c, kStCols, cg.st, [&] {
+            return fft_cols<double>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
+                                    1.0, cg.st);
+        }));
+        HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
+            return rows_phase<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
+                                      scale, image_real, image_imag, accumulate, cg.st);
+        }));
+    } else {
+        HIPCHK(stage(c, kStCols, cg.st, [&] {
+            return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
+                                   1.0, cg.st);
+        }));
+        HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
+            return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp, scale,
+                                     image_real, image_imag, accumulate, cg.st);
+        }));
+    }
+    return LOFAR_OK;
+}
+
+int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,
+                          long polarizations, const double* uu, const double* vv, const double* ww,
+                          const float* weight, double dx, double dw, int w_support,
+                          int grid_size_x, int grid_size_y, const void* cube, void* model_real,
+                          void* model_imag, void* stream)
+{
+    int r = check_grid_args(precision, num_w_planes, num_points, freq_per_chan, polarizations, dx,
+                            dw, w_support, grid_size_x, grid_size_y);
+    if (r) return fail(r);
+    if (!cube || (num_points > 0 && (!uu || !vv || !ww || !weight || !model_real || !model_imag)))
+        return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    const Geometry g =
+        make_geometry(num_w_planes, w_support, grid_size_x, grid_size_y, dx, dw);
+    Buckets b;
+    long long total = 0;
+    r = build_buckets(c, kBucketHome, num_points, uu, vv, ww, g, cg.st, b, total);
+    if (r) return r;
+    r = ensure(c, c.ents, (size_t)total * sizeof(DEntry) + 256);
+    if (r) return r;
+    DEntry* ents = (DEntry*)c.ents.ptr;
+    HIPCHK(stage(c, kStScatter, cg.st,
+                 [&] { return bucket_scatter_home(num_points, uu, vv, ww, g, b, ents, cg.st); }));
+    HIPCHK(stage(c, kStDegrid, cg.st, [&] {
+        return degrid_tiles(precision, g, b, ents, cube, weight, freq_per_chan, polarizations,
+                            model_real, model_imag, cg.st);
+    }));
+    return LOFAR_OK;
+}
+
+int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,
+                     long polarizations, const double* uu, const double* vv, const double* ww,
+                     const float* weight, double dx, double dw, int w_support, int grid_size_x,
+                     int grid_size_y, double resolution, double wmin, double wmax,
+                     const double* image_real, const double* image_imag, void* cube,
+                     void* model_real, void* model_imag, void* stream)
+{
+    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    int r = check_grid_args(precision, num_w_planes, num_points, freq_per_chan, polarizations, dx,
+                            dw, w_support, grid_size_x, grid_size_y);
+    if (r) return fail(r);
+    if (!cube || !image_real || !image_imag) return fail(LOFAR_EARG);
+    {
+        CallGuard cg(stream);
+        if (cg.status) return cg.status;
+        DeviceCtx& c = *cg.c;
+        void *twx, *twy;
+        r = twiddles(c, grid_size_x, precision, &twx);
+        if (r) return r;
+        r = twiddles(c, grid_size_y, precision, &twy);
+        if (r) return r;
+        const PhaseParams pp =
+            phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
+        const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
+        if (precision == LOFAR_FP64) {
+            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
+                return rows_phase_adj<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
+                                              pp, 1.0, image_real, image_imag, cg.st);
+            }));
+            HIPCHK(stage(c, kStCols, cg.st, [&] {
+                return fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                        num_w_planes, scale, cg.st);
+            }));
+        } else {
+            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
+                return rows_phase_adj<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
+                                             pp, 1.0, image_real, image_imag, cg.st);
+            }));
+            HIPCHK(stage(c, kStCols, cg.st, [&] {
+                return fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                       num_w_planes, scale, cg.st);
+            }));
+        }
+    }
+    return lofar_degrid_cube_dev(precision, num_w_planes, num_points, freq_per_chan, polarizations,
+                                 uu, vv, ww, weight, dx, dw, w_support, grid_size_x, grid_size_y,
+                                 cube, model_real, model_imag, stream);
+}
+
+// ---------------------------------------------------------------------------
+// drop-ins: host pointers in, host results out, synchronous
+
+void wstack(int num_w_planes, long num_points, long freq_per_chan, long polarizations,
+            double* uu, double* vv, double* ww, float* vis_real, float* vis_img, float* weight,
+            double dx, double dw, int w_support, int grid_size_x, int grid_size_y, double* grid,
+            int num_threads)
+{
+    (void)num_threads;  // OpenMP thread count of the CPU reference; unused on the GPU
+    t_last_error = LOFAR_OK;
+    int r = check_grid_args(LOFAR_FP64, num_w_planes, num_points, freq_per_chan, polarizations, dx,
+                            dw, w_support, grid_size_x, grid_size_y);
+    if (r || !grid || (num_points > 0 && (!uu || !vv || !ww || !vis_real || !vis_img || !weight))) {
+        fail(r ? r : LOFAR_EARG);
+        report_once("wstack", t_last_error);
+        return;
+    }
+    CallGuard cg(nullptr);
+    if (cg.status) {
+        report_once("wstack", cg.status);
+        return;
+    }
+    DeviceCtx& c = *cg.c;
+    cg.st = c.stream;
+    (void)hipStreamWaitEvent(cg.st, c.last_done, 0);
+    const size_t n = (size_t)num_points, nfp = n * freq_per_chan * polarizations;
+    const size_t cube_bytes = (size_t)2 * num_w_planes * grid_size_x * grid_size_y * sizeof(double);
+    const size_t o_uu = 0, o_vv = align256(o_uu + 8 * n), o_ww = align256(o_vv + 8 * n);
+    const size_t o_vr = align256(o_ww + 8 * n), o_vi = align256(o_vr + 4 * nfp);
+    const size_t o_wt = align256(o_vi + 4 * nfp), o_cube = align256(o_wt + 4 * n * polarizations);
+    r = ensure(c, c.host_io, o_cube + cube_bytes);
+    if (r) {
+        report_once("wstack", r);
+        return;
+    }
+    char* d = (char*)c.host_io.ptr;
+    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+    hipError_t e = hipSuccess;
+    if (n > 0) {
+        e = e ? e : hipMemcpyAsync(d + o_uu, uu, 8 * n, h2d, cg.st);
+        e = e ? e : hipMemcpyAsync(d + o_vv, vv, 8 * n, h2d, cg.st);
+        e = e ? e : hipMemcpyAsync(d + o_ww, ww, 8 * n, h2d, cg.st);
+        e = e ? e : hipMemcpyAsync(d + o_vr, vis_real, 4 * nfp, h2d, cg.st);
+        e = e ? e : hipMemcpyAsync(d + o_vi, vis_img, 4 * nfp, h2d, cg.st);
+        e = e ? e : hipMemcpyAsync(d + o_wt, weight, 4 * n * polarizations, h2d, cg.st);
+    }
+    e = e ? e : hipMemcpyAsync(d + o_cube, grid, cube_bytes, h2d, cg.st);
+    if (e != hipSuccess) {
+        fail(LOFAR_EHIP);
+        report_once("wstack", LOFAR_EHIP);
+        return;
+    }
+    r = grid_impl(c, LOFAR_FP64, 1, num_w_planes, num_points, freq_per_chan, polarizations,
+                  (const double*)(d + o_uu), (const double*)(d + o_vv), (const double*)(d + o_ww),
+                  (const float*)(d + o_vr), (const float*)(d + o_vi), (const float*)(d + o_wt), dx,
+                  dw, w_support, grid_size_x, grid_size_y, d + o_cube, cg.st);
+    if (r) {
+        report_once("wstack", r);
+        return;
+    }
+    e = hipMemcpyAsync(grid, d + o_cube, cube_bytes, hipMemcpyDeviceToHost, cg.st);
+    if (e == hipSuccess) e = hipStreamSynchronize(cg.st);
+    if (e != hipSuccess) {
+        fail(LOFAR_EHIP);
+        report_once("wstack", LOFAR_EHIP);
+    }
+}
+
+void phase_correction(double* gridss, double* image_real, double* image_imag, int xaxis, int yaxis,
+                      int num_w_planes, int xaxistot, int yaxistot, double resolution, double wmin,
+                      double wmax, int num_threads)
+{
+    (void)num_threads;
+    t_last_error = LOFAR_OK;
+    int r = check_phase_args(LOFAR_FP64, xaxis, yaxis, num_w_planes, xaxistot, yaxistot);
+    if (r || !gridss || !image_real || !image_imag) {
+        fail(r ? r : LOFAR_EARG);
+        report_once("phase_correction", t_last_error);
+        return;
+    }
+    CallGuard cg(nullptr);
+    if (cg.status) {
+        report_once("phase_correction", cg.status);
+        return;
+    }
+    DeviceCtx& c = *cg.c;
+    cg.st = c.stream;
+    (void)hipStreamWaitEvent(cg.st, c.last_done, 0);
+    const size_t npix = (size_t)xaxis * yaxis;
+    const size_t cube_bytes = (size_t)num_w_planes * npix * 16;
+    const size_t o_re = align256(cube_bytes), o_im = align256(o_re + 8 * npix);
+    r = ensure(c, c.host_io, o_im + 8 * npix);
+    if (r) {
+        report_once("phase_correction", r);
+        return;
+    }
+    char* d = (char*)c.host_io.ptr;
+    hipError_t e = hipMemcpyAsync(d, gridss, cube_bytes, hipMemcpyHostToDevice, cg.st);
+    e = e ? e : hipMemcpyAsync(d + o_re, image_real, 8 * npix, hipMemcpyHostToDevice, cg.st);
+    e = e ? e : hipMemcpyAsync(d + o_im, image_imag, 8 * npix, hipMemcpyHostToDevice, cg.st);
+    if (e == hipSuccess) {
+        e = stage(c, kStPhase, cg.st, [&] {
+            return phase_sum(LOFAR_FP64, d, (double*)(d + o_re), (double*)(d + o_im), xaxis, yaxis,
+                             num_w_planes, xaxistot, yaxistot, resolution, wmin, wmax, 1, cg.st);
+        });
+    }
+    e = e ? e : hipMemcpyAsync(image_real, d + o_re, 8 * npix, hipMemcpyDeviceToHost, cg.st);
+    e = e ? e : hipMemcpyAsync(image_imag, d + o_im, 8 * npix, hipMemcpyDeviceToHost, cg.st);
+    e = e ? e : hipStreamSynchronize(cg.st);
+    if (e != hipSuccess) {
+        fail(LOFAR_EHIP);
+        report_once("phase_correction", LOFAR_EHIP);
+    }
+}
+
+int lofar_fft_planes(const double* grid, double* gridss, int nx, int ny, int nw)
+{
+    t_last_error = LOFAR_OK;
+    if (!grid || !gridss || nw < 1) return fail(LOFAR_EARG);
+    if (!pow2_ok(nx) || !pow2_ok(ny)) return fail(LOFAR_ESIZE);
+    CallGuard cg(nullptr);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    cg.st = c.stream;
+    (void)hipStreamWaitEvent(cg.st, c.last_done, 0);
+    const size_t bytes = (size_t)nw * nx * ny * 16;
+    int r = ensure(c, c.host_io, bytes);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(c.host_io.ptr, grid, bytes, hipMemcpyHostToDevice, cg.st));
+    r = fft_impl(c, LOFAR_FP64, 1, c.host_io.ptr, c.host_io.ptr, nx, ny, nw,
+                 1.0 / ((double)nx * (double)ny), cg.st);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(gridss, c.host_io.ptr, bytes, hipMemcpyDeviceToHost, cg.st));
+    HIPCHK(hipStreamSynchronize(cg.st));
+    return LOFAR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+// Device building blocks of the Stockham FFT and the host dispatch prototypes.
+#pragma once
+#include "common.h"
+
+namespace lofar {
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+// Per-line plan: E elements per thread, TPL threads per line, radix-16 stages
+// and one radix-2/4/8 remainder stage (last).
+template <int N>
+struct Plan {
+    static constexpr int E = N < 16 ? N : 16;
+    static constexpr int TPL = N / E;
+    static constexpr int LOG = ilog2(N);
+    static constexpr int RL = (N < 16) ? N : ((LOG % 4 == 0) ? 16 : (1 << (LOG % 4)));
+    // element of the line thread t loads into v[q] before the first stage
+    __device__ static __forceinline__ int in_index(int t, int q) { return t + q * TPL; }
+    // element of the line thread t holds in v[e] after the last stage
+    __device__ static __forceinline__ int out_index(int t, int e)
+    {
+        return t + (e / RL) * TPL + (e % RL) * (N / RL);
+    }
+};
+
+// Columns per workgroup of the column pass: about 128 KiB of LDS, at most 1024 threads.
+template <typename Real, int N>
+struct ColW {
+    static constexpr int bytes = N * 2 * (int)sizeof(Real);
+    static constexpr int w0 = 131072 / bytes < 1 ? 1 : (131072 / bytes > 16 ? 16 : 131072 / bytes);
+    static constexpr int wmax = 1024 / Plan<N>::TPL;
+    static constexpr int W = w0 < wmax ? w0 : wmax;
+};
+
+// Spread consecutive workgroups of one XCD (blockIdx % 8) over a contiguous range.
+__device__ __forceinline__ long long xcd_remap(long long b, long long g)
+{
+    if (g % 8 != 0) return b;
+    return (b % 8) * (g / 8) + b / 8;
+}
+
+// w-phase geometry of phase_correction.cu:92-94,164-172
+struct PhaseParams {
+    int xtot, ytot;
+    double resolution, wmin, dw, dwnorm;
+    int phase_on;
+    __device__ __forceinline__ double coord(int i, int tot) const
+    {
+        double c = (double)(i - tot / 2);
+        if (c < 0.0) c = (double)(i + tot / 2);
+        return sin(c * resolution);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// complex arithmetic
+
+template <typename T2>
+__device__ __forceinline__ T2 c_add(T2 a, T2 b)
+{
+    return T2{a.x + b.x, a.y + b.y};
+}
+template <typename T2>
+__device__ __forceinline__ T2 c_sub(T2 a, T2 b)
+{
+    return T2{a.x - b.x, a.y - b.y};
+}
+// a * w, or a * conj(w) when SIGN < 0
+template <int SIGN, typename T2>
+__device__ __forceinline__ T2 c_mul_tw(T2 a, T2 w)
+{
+    if constexpr (SIGN > 0)
+        return T2{a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
+    else
+        return T2{a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
+}
+// a * (S*i)
+template <int S, typename T2>
+__device__ __forceinline__ T2 c_muli(T2 a)
+{
+    if constexpr (S > 0)
+        return T2{-a.y, a.x};
+    else
+        return T2{a.y, -a.x};
+}
+
+// exp(2*pi*i*m/16) components; values of cos(pi/8), cos(pi/4), cos(3*pi/8)
+constexpr double kC1 = 0.92387953251128675613;
+constexpr double kC2 = 0.70710678118654752440;
+constexpr double kC3 = 0.38268343236508977173;
+constexpr double cos16(int m)
+{
+    m &= 15;
+    return m == 0 ? 1.0 : m == 1 ? kC1 : m == 2 ? kC2 : m == 3 ? kC3 : m == 4 ? 0.0
+         : m == 5 ? -kC3 : m == 6 ? -kC2 : m == 7 ? -kC1 : m == 8 ? -1.0 : m == 9 ? -kC1
+         : m == 10 ? -kC2 : m == 11 ? -kC3 : m == 12 ? 0.0 : m == 13 ? kC3 : m == 14 ? kC2 : kC1;
+}
+constexpr double sin16(int m) { return cos16(m - 4); }
+
+// a * exp(S*2*pi*i*M/16)
+template <int M, int S, typename T2>
+__device__ __forceinline__ T2 c_w16(T2 a)
+{
+    constexpr int m = M & 15;
+    if constexpr (m == 0)
+        return a;
+    else if constexpr (m == 4)
+        return c_muli<S>(a);
+    else if constexpr (m == 8)
+        return T2{-a.x, -a.y};
+    else if constexpr (m == 12)
+        return c_muli<-S>(a);
+    else {
+        using R = decltype(a.x);
+        constexpr R c = (R)cos16(m);
+        constexpr R s = (R)(S * sin16(m));
+        return T2{a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+}
+
+template <int S, typename T2>
+__device__ __forceinline__ void dft4(T2& x0, T2& x1, T2& x2, T2& x3)
+{
+    const T2 t0 = c_add(x0, x2), t1 = c_sub(x0, x2), t2 = c_add(x1, x3);
+    const T2 t3 = c_muli<S>(c_sub(x1, x3));
+    x0 = c_add(t0, t2);
+    x2 = c_sub(t0, t2);
+    x1 = c_add(t1, t3);
+    x3 = c_sub(t1, t3);
+}
+
+template <int N1, int S, typename T2>
+__device__ __forceinline__ void tw16_col(T2* v)
+{
+    // v[4*k1 + N1] *= W16^(N1*k1), k1 = 1..3
+    v[4 + N1] = c_w16<N1 * 1, S>(v[4 + N1]);
+    v[8 + N1] = c_w16<N1 * 2, S>(v[8 + N1]);
+    v[12 + N1] = c_w16<N1 * 3, S>(v[12 + N1]);
+}
+
+// In-register DFT of size R (natural order in and out), X_k = sum x_n exp(S*2*pi*i*n*k/R).
+template <int R, int S, typename T2>
+__device__ __forceinline__ void dft(T2* v)
+{
+    if constexpr (R == 2) {
+        const T2 a = v[0];
+        v[0] = c_add(a, v[1]);
+        v[1] = c_sub(a, v[1]);
+    } else if constexpr (R == 4) {
+        dft4<S>(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        // n = 2*n1 + n2: radix-4 over n1, twiddle W8^(n2*k1), radix-2 over n2
+        dft4<S>(v[0], v[2], v[4], v[6]);
+        dft4<S>(v[1], v[3], v[5], v[7]);
+        v[3] = c_w16<2, S>(v[3]);
+        v[5] = c_w16<4, S>(v[5]);
+        v[7] = c_w16<6, S>(v[7]);
+        T2 o[8];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            o[k1] = c_add(v[2 * k1], v[2 * k1 + 1]);
+            o[k1 + 4] = c_sub(v[2 * k1], v[2 * k1 + 1]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = o[k];
+    } else if constexpr (R == 16) {
+        // n = 4*n1 + n2: radix-4 over n1, twiddle W16^(n2*k1), radix-4 over n2
+        dft4<S>(v[0], v[4], v[8], v[12]);
+        dft4<S>(v[1], v[5], v[9], v[13]);
+        dft4<S>(v[2], v[6], v[10], v[14]);
+        dft4<S>(v[3], v[7], v[11], v[15]);
+        tw16_col<1, S>(v);
+        tw16_col<2, S>(v);
+        tw16_col<3, S>(v);
+        dft4<S>(v[0], v[1], v[2], v[3]);
+        dft4<S>(v[4], v[5], v[6], v[7]);
+        dft4<S>(v[8], v[9], v[10], v[11]);
+        dft4<S>(v[12], v[13], v[14], v[15]);
+        T2 o[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = o[k];
+    }
+}
+
+// Stockham stages from span NS on.  On entry v[m*R+q] holds element b+q*(N/R)
+// of butterfly b = t + m*TPL; on return v holds the outputs in the order of
+// Plan<N>::out_index.  `lds` addresses element i of this line at lds[i*ls].
+// Twiddles: tw[m] = exp(+2*pi*i*m/N), conjugated for SIGN < 0.
+template <typename T2, int N, int SIGN, int NS>
+__device__ __forceinline__ void stages(T2 (&v)[Plan<N>::E], T2* lds, int ls, int t,
+                                       const T2* __restrict__ tw)
+{
+    constexpr int E = Plan<N>::E;
+    constexpr int TPL = Plan<N>::TPL;
+    constexpr int R = (N / NS >= 16) ? 16 : N / NS;
+    constexpr int M = E / R;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int b = t + m * TPL;
+        if constexpr (NS > 1) {
+            const int k = b & (NS - 1);
+#pragma unroll
+            for (int q = 1; q < R; q++)
+                v[m * R + q] = c_mul_tw<SIGN>(v[m * R + q], tw[q * k * (N / (NS * R))]);
+        }
+        dft<R, SIGN>(&v[m * R]);
+    }
+    if constexpr (NS * R < N) {
+        constexpr int NS2 = NS * R;
+        constexpr int R2 = (N / NS2 >= 16) ? 16 : N / NS2;
+        constexpr int M2 = E / R2;
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int b = t + m * TPL;
+            const int k = b & (NS - 1);
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) lds[(base + q * NS) * ls] = v[m * R + q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M2; m++) {
+            const int b = t + m * TPL;
+#pragma unroll
+            for (int q = 0; q < R2; q++) v[m * R2 + q] = lds[(b + q * (N / R2)) * ls];
+        }
+        stages<T2, N, SIGN, NS2>(v, lds, ls, t, tw);
+    }
+}
+
+// phase constants of phase_correction.cu:92-94 (phase.hip)
+PhaseParams phase_params(int nw, int xtot, int ytot, double resolution, double wmin, double wmax);
+
+// host dispatch (fft.hip); all return hipErrorInvalidValue for unsupported sizes
+template <typename Real>
+hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx, long long lines,
+                    double scale, hipStream_t st);
+template <typename Real>
+hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx, int ny, int nw,
+                    double scale, hipStream_t st);
+template <typename Real>
+hipError_t rows_phase(const void* cube, const void* tw, int nx, int ny, int nw,
+                      const PhaseParams& pp, double scale, double* re, double* im, int accumulate,
+                      hipStream_t st);
+template <typename Real>
+hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
+                          const PhaseParams& pp, double scale, const double* re, const double* im,
+                          hipStream_t st);
+
+}  // namespace lofar

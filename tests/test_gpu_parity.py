@@ -1,0 +1,333 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the reference's
+golden fixtures, at sizes the oracle finishes in seconds.
+
+Tolerances (relative L2 over the whole array unless stated):
+  FP64 mode: cube 1e-12, plane transform 1e-13, image 1e-10 (BASELINE.json target);
+  FP32C mode: cube 1e-6, plane transform 2e-6, image 1e-6.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def T(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def dev_inputs(d, dev):
+    return [T(d[k], dev) for k in ("uu", "vv", "ww", "vis_real", "vis_img", "weight")]
+
+
+def random_case(seed, n, nx, ny, nw, F=1, P=1, edge=True, nan=0.02):
+    rng = np.random.default_rng(seed)
+    if edge:
+        u = rng.uniform(-0.99, nx + 3.0, n) / nx
+        v = rng.uniform(-0.99, ny + 3.0, n) / nx
+    else:
+        u = rng.uniform(0.05, 0.95, n)
+        v = rng.uniform(0.05, 0.95, n) * ny / nx
+    w = rng.uniform(-0.99 / nw, 1 - 1e-12, n)
+    vr = rng.standard_normal(n * F * P).astype(np.float32)
+    vi = rng.standard_normal(n * F * P).astype(np.float32)
+    if nan:
+        vr[rng.random(n * F * P) < nan] = np.nan
+    wt = rng.uniform(0.2, 2.0, n * P).astype(np.float32)
+    return dict(uu=u, vv=v, ww=w, vis_real=vr, vis_img=vi, weight=wt)
+
+
+# ---------------------------------------------------------------------------
+# gridding
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))),
+                         ids=lambda p: os.path.basename(p)[:-4])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_grid_matches_reference_fixture(cuda, path, precision, tol):
+    from lofar_amd import imager
+    z = np.load(path)
+    nx, ny, nw, F, P = (int(z[k]) for k in ("nx", "ny", "nw", "F", "P"))
+    cube = imager.grid(*dev_inputs(z, cuda), nw, int(z["support"]), nx, ny, precision=precision,
+                       freq_per_chan=F, polarizations=P)
+    assert rel(H(cube), z["grid"]) < tol
+
+
+@pytest.mark.parametrize("seed,n,nx,ny,nw,support,F,P,edge", [
+    (1, 20000, 128, 128, 4, 7, 1, 1, False),
+    (2, 20000, 96, 200, 3, 7, 1, 1, True),
+    (3, 5000, 64, 64, 2, 9, 2, 2, True),
+    (4, 5000, 130, 70, 5, 3, 1, 1, True),
+    (5, 3000, 40, 40, 1, 1, 1, 1, True),
+    (6, 3000, 64, 64, 3, 17, 1, 1, True),
+    (7, 200000, 256, 256, 8, 7, 1, 1, False),
+])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_grid_matches_oracle(cuda, seed, n, nx, ny, nw, support, F, P, edge, precision, tol):
+    from lofar_amd import imager
+    d = random_case(seed, n, nx, ny, nw, F, P, edge)
+    st, want = O.wstack(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / nx, 1 / nw, support, nx, ny, freq_per_chan=F, polarizations=P)
+    assert st == O.OK
+    cube = imager.grid(*dev_inputs(d, cuda), nw, support, nx, ny, precision=precision,
+                       freq_per_chan=F, polarizations=P)
+    assert rel(H(cube), want) < tol
+
+
+def test_grid_accumulate_and_empty(cuda):
+    import torch
+    from lofar_amd import imager
+    d = random_case(8, 4000, 64, 64, 2)
+    c1 = imager.grid(*dev_inputs(d, cuda), 2, 7, 64, 64)
+    c2 = imager.grid(*dev_inputs(d, cuda), 2, 7, 64, 64, cube=c1.clone(), accumulate=True)
+    assert rel(H(c2), 2 * H(c1)) < 1e-14
+    empty = [t[:0] for t in dev_inputs(d, cuda)]
+    c3 = imager.grid(*empty, 2, 7, 64, 64, cube=torch.full_like(c1, 3.0))
+    assert not H(c3).any()
+
+
+def test_grid_rejects_bad_w_and_writes_nothing(cuda):
+    import torch
+    from lofar_amd import imager
+    from lofar_amd._lib import ECOORD, LofarError
+    d = random_case(9, 1000, 32, 32, 2)
+    d["ww"][17] = 1.0  # (int)(1.0/0.5) = 2 = Nw: outside the cube
+    cube = torch.full((2, 32, 32), 5.0, dtype=torch.complex128, device=cuda)
+    with pytest.raises(LofarError) as e:
+        imager.grid(*dev_inputs(d, cuda), 2, 7, 32, 32, cube=cube)
+    assert e.value.code == ECOORD
+    torch.cuda.synchronize()
+    assert (H(cube) == 5.0).all()
+
+
+def test_wstack_dropin_accumulates_like_reference(cuda):
+    from lofar_amd import imager
+    z = np.load(os.path.join(GOLDEN, "nan_fp2_s5.npz"))
+    nx, ny, nw, F, P = (int(z[k]) for k in ("nx", "ny", "nw", "F", "P"))
+    n = len(z["uu"])
+    start = np.random.default_rng(0).standard_normal(2 * nw * nx * ny)
+    grid = start.copy()
+    imager.wstack(nw, n, F, P, z["uu"], z["vv"], z["ww"], z["vis_real"], z["vis_img"], z["weight"],
+                  1 / nx, 1 / nw, int(z["support"]), nx, ny, grid)
+    want = start + z["grid"].view(np.float64).ravel()
+    assert rel(grid, want) < 1e-12
+
+
+def test_wstack_dropin_error_leaves_grid(cuda):
+    from lofar_amd import imager
+    from lofar_amd._lib import ECOORD, LofarError
+    d = random_case(10, 100, 16, 16, 2)
+    d["ww"][3] = float("nan")
+    grid = np.full(2 * 2 * 16 * 16, 7.0)
+    with pytest.raises(LofarError) as e:
+        imager.wstack(2, 100, 1, 1, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"],
+                      d["weight"], 1 / 16, 1 / 2, 7, 16, 16, grid)
+    assert e.value.code == ECOORD and (grid == 7.0).all()
+
+
+# ---------------------------------------------------------------------------
+# plane transform
+
+@pytest.mark.parametrize("nx,ny,nw", [(2, 2, 3), (4, 16, 2), (16, 16, 2), (32, 8, 1), (64, 64, 3),
+                                      (128, 512, 1), (256, 256, 2), (1024, 1024, 1),
+                                      (2048, 64, 1), (64, 2048, 1), (4096, 4096, 1)])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_fft_matches_numpy(cuda, nx, ny, nw, sign):
+    from lofar_amd import imager
+    rng = np.random.default_rng(nx * 7 + ny + nw)
+    x = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    want = np.fft.ifft2(x, axes=(1, 2)) if sign > 0 else np.fft.fft2(x, axes=(1, 2)) / (nx * ny)
+    got64 = H(imager.fft(T(x, cuda), sign=sign))
+    assert rel(got64, want) < 1e-13
+    got32 = H(imager.fft(T(x.astype(np.complex64), cuda), sign=sign))
+    assert rel(got32, want) < 2e-6
+
+
+def test_fft_8192_single_plane(cuda):
+    from lofar_amd import imager
+    rng = np.random.default_rng(81)
+    x = (rng.standard_normal((1, 8192, 8192)) + 1j * rng.standard_normal((1, 8192, 8192))).astype(np.complex64)
+    want = np.fft.ifft2(x.astype(np.complex128), axes=(1, 2))
+    assert rel(H(imager.fft(T(x, cuda))), want) < 2e-6
+    assert rel(H(imager.fft(T(x.astype(np.complex128), cuda))), want) < 1e-13
+
+
+def test_fft_in_place_and_rejects_non_pow2(cuda):
+    from lofar_amd import imager
+    from lofar_amd._lib import ESIZE, LofarError
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 64, 32)) + 1j * rng.standard_normal((2, 64, 32))
+    t = T(x, cuda)
+    imager.fft(t, out=t)
+    assert rel(H(t), np.fft.ifft2(x, axes=(1, 2))) < 1e-13
+    with pytest.raises(LofarError) as e:
+        imager.fft(T(np.zeros((1, 12, 16), complex), cuda))
+    assert e.value.code == ESIZE
+
+
+def test_fftw_data_dropin(cuda):
+    from lofar_amd import imager
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((3, 128, 64)) + 1j * rng.standard_normal((3, 128, 64))
+    out = np.zeros(2 * x.size)
+    imager.fftw_data(x.view(np.float64).ravel(), out, 64, 128, 3)
+    assert rel(out.view(np.complex128).reshape(x.shape), O.fft_planes(x)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# phase correction
+
+@pytest.mark.parametrize("nx,ny,nw,wmin,wmax", [(64, 64, 8, -300.0, 400.0), (128, 32, 1, -5.0, 5.0),
+                                                (32, 96, 5, 0.0, 1200.0)])
+def test_phase_dropin_matches_oracle(cuda, nx, ny, nw, wmin, wmax):
+    from lofar_amd import imager
+    rng = np.random.default_rng(nx + ny + nw)
+    cube = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    res = 1 / (2.0 * nx)
+    re0 = rng.standard_normal(nx * ny)
+    im0 = rng.standard_normal(nx * ny)
+    re, im = re0.copy(), im0.copy()
+    imager.phase_correction(cube.view(np.float64).ravel(), re, im, nx, ny, nw, nx, ny, res, wmin, wmax)
+    want = O.phase_correction(cube, res, wmin, wmax, image=(re0 + 1j * im0).reshape(ny, nx))
+    assert rel((re + 1j * im).reshape(ny, nx), want) < 1e-12
+    # device entry point, both precisions, overwrite semantics
+    r, i = imager.phase(T(cube, cuda), res, wmin, wmax)
+    want0 = O.phase_correction(cube, res, wmin, wmax)
+    assert rel(H(r) + 1j * H(i), want0) < 1e-12
+    r, i = imager.phase(T(cube.astype(np.complex64), cuda), res, wmin, wmax)
+    assert rel(H(r) + 1j * H(i), O.phase_correction(cube.astype(np.complex64), res, wmin, wmax)) < 1e-12
+
+
+def test_phase_uses_global_axes_for_coordinates(cuda):
+    # xaxistot/yaxistot drive the coordinate shift (phase_correction.cu:164-169)
+    from lofar_amd import imager
+    rng = np.random.default_rng(6)
+    nx, ny, nw = 32, 16, 3
+    cube = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    re, im = np.zeros(nx * ny), np.zeros(nx * ny)
+    imager.phase_correction(cube.view(np.float64).ravel(), re, im, nx, ny, nw, nx, 64, 1 / 64, -9.0, 11.0)
+    want = np.zeros((ny, nx), complex)
+    lo = O.lib()
+    wr, wi = np.zeros(nx * ny), np.zeros(nx * ny)
+    lo.oracle_phase_correction(O._d(cube.view(np.float64).ravel()), O._d(wr), O._d(wi), nx, ny, nw,
+                               nx, 64, 1 / 64, -9.0, 11.0, 1)
+    want = (wr + 1j * wi).reshape(ny, nx)
+    assert rel((re + 1j * im).reshape(ny, nx), want) < 1e-12
+
+
+def test_phase_adjoint_matches_oracle(cuda):
+    from lofar_amd import imager
+    rng = np.random.default_rng(7)
+    ny, nx, nw = 48, 64, 6
+    img = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    want = O.phase_adjoint(img, nw, 1 / 128, -70.0, 90.0)
+    got = imager.phase_adjoint(T(img.real.copy(), cuda), T(img.imag.copy(), cuda), nw, 1 / 128, -70.0, 90.0)
+    assert rel(H(got), want) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# whole hot path and its adjoint
+
+def synth(n, nx, nw, seed):
+    from lofar_amd.synth import make_obs
+    return make_obs(n, nx, nw, seed=seed)
+
+
+@pytest.mark.parametrize("n,nx,nw,support", [(30000, 256, 4, 7), (1 << 20, 1024, 8, 7),
+                                             (50000, 512, 16, 9)])
+def test_image_pipeline_matches_oracle(cuda, n, nx, nw, support):
+    from lofar_amd import imager
+    d = synth(n, nx, nw, seed=n + nx)
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], support,
+                   nx, nx, d["resolution"], d["wmin"], d["wmax"])
+    r, i = imager.image(*dev_inputs(d, cuda), nw, support, nx, nx, d["resolution"], d["wmin"],
+                        d["wmax"], precision="fp64")
+    e64 = rel(H(r) + 1j * H(i), want)
+    r, i = imager.image(*dev_inputs(d, cuda), nw, support, nx, nx, d["resolution"], d["wmin"],
+                        d["wmax"], precision="fp32c")
+    e32 = rel(H(r) + 1j * H(i), want)
+    print("image rel err fp64 %.3e fp32c %.3e" % (e64, e32))
+    assert e64 < 1e-10
+    assert e32 < 1e-6
+
+
+def test_dropin_chain_matches_oracle(cuda):
+    # wstack -> fftw_data -> phase_correction through the host drop-ins, as gridding.c,
+    # fourier_transform.c:7-74 and :173 call them
+    from lofar_amd import imager
+    d = synth(40000, 256, 8, seed=11)
+    nx = ny = 256
+    nw = 8
+    grid = np.zeros(2 * nw * nx * ny)
+    imager.wstack(nw, len(d["uu"]), 1, 1, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"],
+                  d["weight"], 1 / nx, 1 / nw, 7, nx, ny, grid)
+    gridss = np.zeros_like(grid)
+    imager.fftw_data(grid, gridss, nx, ny, nw)
+    re, im = np.zeros(nx * ny), np.zeros(nx * ny)
+    imager.phase_correction(gridss, re, im, nx, ny, nw, nx, ny, d["resolution"], d["wmin"], d["wmax"])
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7, nx, ny,
+                   d["resolution"], d["wmin"], d["wmax"])
+    assert rel((re + 1j * im).reshape(ny, nx), want) < 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-5)])
+def test_degrid_cube_matches_oracle(cuda, precision, tol):
+    from lofar_amd import imager
+    d = random_case(12, 30000, 128, 96, 4, edge=True, nan=0)
+    rng = np.random.default_rng(13)
+    cube = rng.standard_normal((4, 96, 128)) + 1j * rng.standard_normal((4, 96, 128))
+    st, want = O.degrid(cube, d["uu"], d["vv"], d["ww"], d["weight"], 1 / 128, 1 / 4, 7)
+    c = T(cube if precision == "fp64" else cube.astype(np.complex64), cuda)
+    mr, mi = imager.degrid_cube(c, T(d["uu"], cuda), T(d["vv"], cuda), T(d["ww"], cuda),
+                                T(d["weight"], cuda), 7)
+    assert st == O.OK and rel(H(mr) + 1j * H(mi), want) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-5)])
+def test_degrid_matches_oracle_adjoint(cuda, precision, tol):
+    from lofar_amd import imager
+    d = synth(20000, 256, 8, seed=14)
+    rng = np.random.default_rng(15)
+    img = rng.standard_normal((256, 256)) + 1j * rng.standard_normal((256, 256))
+    want = O.degrid_image(img, 8, d["uu"], d["vv"], d["ww"], d["weight"], 7, d["resolution"],
+                          d["wmin"], d["wmax"])
+    mr, mi = imager.degrid(T(img.real.copy(), cuda), T(img.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), 8, 7,
+                           d["resolution"], d["wmin"], d["wmax"], precision=precision)
+    assert rel(H(mr) + 1j * H(mi), want) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_adjoint_dot_test(cuda, precision, tol):
+    import torch
+    from lofar_amd import imager
+    d = synth(100000, 512, 8, seed=16)
+    x = d["vis_real"].astype(np.float64) + 1j * d["vis_img"].astype(np.float64)
+    r, i = imager.image(*dev_inputs(d, cuda), 8, 7, 512, 512, d["resolution"], d["wmin"], d["wmax"],
+                        precision=precision)
+    ax = H(r) + 1j * H(i)
+    rng = np.random.default_rng(17)
+    y = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
+    mr, mi = imager.degrid(T(y.real.copy(), cuda), T(y.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), 8, 7,
+                           d["resolution"], d["wmin"], d["wmax"], precision=precision)
+    ahy = H(mr).astype(np.float64) + 1j * H(mi).astype(np.float64)
+    lhs, rhs = np.vdot(y, ax), np.vdot(ahy, x)
+    print("dot test %s: %.3e" % (precision, abs(lhs - rhs) / abs(lhs)))
+    assert abs(lhs - rhs) < tol * abs(lhs)
+    torch.cuda.synchronize()

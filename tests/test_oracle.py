@@ -1,0 +1,270 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+* against the reference's own C compiled in place (oracle/_ref, when built) and
+  the golden fixtures that C produced (tests/golden/*.npz);
+* against numpy.fft and a brute-force DFT for the plane transform;
+* against closed forms (one sample's footprint, window clipping at both edges,
+  truncation of negative coordinates, NaN skip, weight indexing, the phase at the
+  centre pixel, the single-plane sum, integer-abs resolution);
+* the adjoint pairs against the dot-product identity.
+"""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF = O.RefLib(fast=False)
+needs_ref = pytest.mark.skipif(not REF.available, reason="oracle/_ref not built (reference sources absent)")
+PI_REF = 3.14159265359
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    return float(np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def golden_cases():
+    return sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+@pytest.mark.parametrize("path", golden_cases(), ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_wstack_equals_reference_fixture(path):
+    z = np.load(path)
+    nx, ny, nw, F, P = (int(z[k]) for k in ("nx", "ny", "nw", "F", "P"))
+    st, grid = O.wstack(nw, z["uu"], z["vv"], z["ww"], z["vis_real"], z["vis_img"], z["weight"],
+                        1.0 / nx, 1.0 / nw, int(z["support"]), nx, ny, freq_per_chan=F,
+                        polarizations=P)
+    assert st == O.OK
+    # same expression order, no FMA contraction on either side: bit-identical
+    assert np.array_equal(grid, z["grid"])
+
+
+@pytest.mark.parametrize("path", golden_cases(), ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_pipeline_matches_reference_fixture(path):
+    z = np.load(path)
+    planes = O.fft_planes(z["grid"])
+    img = O.phase_correction(planes, float(z["res"]), float(z["wmin"]), float(z["wmax"]))
+    assert rel(img, z["image"]) < 1e-13
+
+
+def _random_case(seed, n, nx, ny, nw, F=1, P=1, edge=True):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-0.99, nx + 3.0, n) / nx
+    v = rng.uniform(-0.99, ny + 3.0, n) / nx
+    if not edge:
+        u = rng.uniform(0.2, 0.8, n)
+        v = rng.uniform(0.2, 0.8, n) * ny / nx
+    w = rng.uniform(-0.99 / nw, 1 - 1e-12, n)
+    vr = rng.standard_normal(n * F * P).astype(np.float32)
+    vi = rng.standard_normal(n * F * P).astype(np.float32)
+    vr[rng.random(n * F * P) < 0.05] = np.nan
+    wt = rng.uniform(0.1, 3, n * P).astype(np.float32)
+    return u, v, w, vr, vi, wt
+
+
+@needs_ref
+@pytest.mark.parametrize("seed,n,nx,ny,nw,support,F,P", [
+    (10, 500, 32, 32, 4, 7, 1, 1), (11, 500, 64, 16, 2, 9, 1, 2), (12, 300, 16, 40, 3, 3, 3, 1),
+    (13, 200, 8, 8, 5, 7, 2, 2), (14, 400, 32, 32, 1, 1, 1, 1), (15, 300, 24, 30, 2, 6, 1, 1)])
+def test_oracle_wstack_equals_reference_build(seed, n, nx, ny, nw, support, F, P):
+    u, v, w, vr, vi, wt = _random_case(seed, n, nx, ny, nw, F, P)
+    st, g0 = O.wstack(nw, u, v, w, vr, vi, wt, 1 / nx, 1 / nw, support, nx, ny, freq_per_chan=F,
+                      polarizations=P)
+    assert st == O.OK
+    g1 = REF.wstack(nw, u, v, w, vr, vi, wt, 1 / nx, 1 / nw, support, nx, ny, freq_per_chan=F,
+                    polarizations=P, variant="serial")
+    assert np.array_equal(g0, g1)
+    g2 = REF.wstack(nw, u, v, w, vr, vi, wt, 1 / nx, 1 / nw, support, nx, ny, freq_per_chan=F,
+                    polarizations=P, variant="omp", num_threads=1)
+    assert np.array_equal(g0, g2)
+    g3 = REF.wstack(nw, u, v, w, vr, vi, wt, 1 / nx, 1 / nw, support, nx, ny, freq_per_chan=F,
+                    polarizations=P, variant="omp", num_threads=4)
+    assert rel(g3, g0) < 1e-14
+
+
+@needs_ref
+@pytest.mark.parametrize("nx,ny,nw,wmin,wmax", [(32, 32, 4, -50.0, 70.0), (16, 24, 1, -3.0, 3.0),
+                                                (64, 32, 7, 0.0, 900.0)])
+def test_oracle_phase_equals_reference_build(nx, ny, nw, wmin, wmax):
+    rng = np.random.default_rng(nx * ny + nw)
+    cube = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    res = 1.0 / (2 * nx)
+    a = O.phase_correction(cube, res, wmin, wmax)
+    b = REF.phase_correction(cube, res, wmin, wmax, num_threads=1)
+    assert np.array_equal(a, b)
+    # accumulate semantics: a second call adds
+    c = REF.phase_correction(cube, res, wmin, wmax, num_threads=1, image=b.copy())
+    assert rel(c, 2 * b) < 1e-14
+
+
+@needs_ref
+def test_reference_test_function():
+    assert REF.serial.test(41) == 42 and REF.omp.test(-1) == 0
+
+
+# ---------------------------------------------------------------------------
+# plane transform
+
+@pytest.mark.parametrize("nx,ny,nw", [(2, 2, 1), (4, 8, 2), (16, 16, 3), (64, 32, 2), (128, 256, 1)])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_fft_matches_numpy(nx, ny, nw, sign):
+    rng = np.random.default_rng(nx + 3 * ny)
+    cube = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    got = O.fft_planes(cube, sign=sign, scale=1.0)
+    want = np.fft.ifft2(cube, axes=(1, 2)) * nx * ny if sign > 0 else np.fft.fft2(cube, axes=(1, 2))
+    assert rel(got, want) < 1e-14
+
+
+def test_fft_non_power_of_two_bruteforce():
+    rng = np.random.default_rng(5)
+    nx, ny = 6, 10
+    x = rng.standard_normal((1, ny, nx)) + 1j * rng.standard_normal((1, ny, nx))
+    got = O.fft_planes(x)[0]
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    want = np.zeros((ny, nx), complex)
+    for v in range(ny):
+        for u in range(nx):
+            want += x[0, v, u] * np.exp(2j * np.pi * (u * xx / nx + v * yy / ny))
+    assert rel(got, want / (nx * ny)) < 1e-14
+
+
+def test_fft_delta_closed_form():
+    nx, ny = 16, 8
+    cube = np.zeros((1, ny, nx), complex)
+    cube[0, 3, 5] = 1.0
+    got = O.fft_planes(cube)[0]
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    want = np.exp(2j * np.pi * (5 * xx / nx + 3 * yy / ny)) / (nx * ny)
+    assert np.max(np.abs(got - want)) < 1e-16
+
+
+# ---------------------------------------------------------------------------
+# gridding closed forms
+
+def _one(pu, pv, pw, nx=16, ny=16, nw=3, support=7, vis=(1.5, -0.5), wgt=2.0):
+    st, g = O.wstack(nw, np.array([pu / nx]), np.array([pv / nx]), np.array([pw / nw]),
+                     np.array([vis[0]], np.float32), np.array([vis[1]], np.float32),
+                     np.array([wgt], np.float32), 1 / nx, 1 / nw, support, nx, ny)
+    return st, g
+
+
+def _footprint(pu, pv, nx, ny, K, vis, wgt):
+    out = {}
+    gu, gv = int(pu), int(pv)
+    for k in range(max(gv - K, 0), min(gv + K, ny - 1) + 1):
+        for j in range(max(gu - K, 0), min(gu + K, nx - 1) + 1):
+            cw = 0.5 / PI_REF * math.exp(-((j + 0.5 - pu) ** 2 + (k + 0.5 - pv) ** 2) * 0.5)
+            out[(k, j)] = complex(np.float32(wgt) * np.float32(vis[0]),
+                                  np.float32(wgt) * np.float32(vis[1])) * cw
+    return out
+
+
+@pytest.mark.parametrize("pu,pv", [(7.3, 8.6), (0.3, 0.2), (15.6, 15.9), (17.5, 3.2), (-0.7, 9.1),
+                                   (13.0, 2.0), (3.0, 12.999)])
+def test_single_sample_footprint(pu, pv):
+    nx = ny = 16
+    st, g = _one(pu, pv, 1.4)
+    assert st == O.OK
+    want = _footprint(pu, pv, nx, ny, 3, (1.5, -0.5), 2.0)
+    assert np.count_nonzero(g[0]) == 0 and np.count_nonzero(g[2]) == 0  # plane (int)1.4 = 1
+    for (k, j), val in want.items():
+        assert abs(g[1, k, j] - val) <= 1e-15 * abs(val)
+    assert np.count_nonzero(g[1]) == len(want)
+
+
+def test_truncation_and_plane_edges():
+    # w in (-dw, 0) truncates to plane 0; w just below 1 is the last plane
+    st, g = _one(5.5, 5.5, -0.99)
+    assert st == O.OK and np.count_nonzero(g[0]) == 49
+    st, g = _one(5.5, 5.5, 3 - 1e-9)
+    assert st == O.OK and np.count_nonzero(g[2]) == 49
+    for bad in (-1.0, 3.0, float("nan"), float("inf")):
+        st, g = _one(5.5, 5.5, bad)
+        assert st == O.ECOORD and not g.any()
+
+
+def test_nan_real_part_skips_sample():
+    nx = 16
+    u = np.array([0.4, 0.4]); v = np.array([0.5, 0.5]); w = np.array([0.1, 0.1])
+    vr = np.array([np.nan, 1.0], np.float32); vi = np.array([5.0, 0.0], np.float32)
+    st, g = O.wstack(1, u, v, w, vr, vi, np.ones(2, np.float32), 1 / nx, 1.0, 7, nx, nx)
+    st1, g1 = O.wstack(1, u[1:], v[1:], w[1:], vr[1:], vi[1:], np.ones(1, np.float32), 1 / nx, 1.0, 7, nx, nx)
+    assert st == st1 == O.OK and np.array_equal(g, g1)
+
+
+def test_weight_index_resets_per_channel():
+    # F=2, P=2: weight index is i*P+p for every channel (w-stacking_omp.c:120)
+    nx = 16
+    vr = np.array([1, 2, 3, 4], np.float32); vi = np.zeros(4, np.float32)
+    wt = np.array([10, 100], np.float32)
+    st, g = O.wstack(1, np.array([0.5]), np.array([0.5]), np.array([0.0]), vr, vi, wt, 1 / nx, 1.0,
+                     1, nx, nx, freq_per_chan=2, polarizations=2)
+    cw = 0.5 / PI_REF * math.exp(-(0.0 ** 2 + 0.0 ** 2) * 0.5)
+    # pos = 8.0 -> cell 8 with distance 0.5 in u and v
+    cw = 0.5 / PI_REF * math.exp(-(0.5 ** 2 + 0.5 ** 2) * 0.5)
+    want = (10 * 1 + 100 * 2 + 10 * 3 + 100 * 4) * cw
+    assert st == O.OK and abs(g[0, 8, 8].real - want) < 1e-12 * want
+
+
+def test_accumulates_into_grid():
+    u, v, w, vr, vi, wt = _random_case(3, 100, 16, 16, 2, edge=False)
+    st, g = O.wstack(2, u, v, w, vr, vi, wt, 1 / 16, 1 / 2, 7, 16, 16)
+    st, g2 = O.wstack(2, u, v, w, vr, vi, wt, 1 / 16, 1 / 2, 7, 16, 16, grid=g.copy())
+    assert np.allclose(g2, 2 * g, rtol=1e-15, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# phase closed forms
+
+def test_phase_centre_pixel_and_single_plane():
+    nx = ny = 16
+    rng = np.random.default_rng(2)
+    cube = rng.standard_normal((4, ny, nx)) + 1j * rng.standard_normal((4, ny, nx))
+    img = O.phase_correction(cube, 1 / 32, -20.0, 30.0)
+    # pixel (nx/2, ny/2): X = Y = 0 -> r = 0, phase 0, factor dwnorm = 1/Nw
+    assert abs(img[ny // 2, nx // 2] - cube[:, ny // 2, nx // 2].sum() / 4) < 1e-15
+    one = O.phase_correction(cube[:1], 1 / 32, -20.0, 30.0)
+    assert np.array_equal(one, cube[0])  # Nw == 1: plain sum, no factor
+
+
+def test_resolution_uses_integer_abs():
+    assert O.resolution(-3.7, 2.9) == 1.0 / 3
+    assert O.resolution(0.0, 8192.0) == 1.0 / 8192
+    assert math.isinf(O.resolution(0.4, -0.9))
+
+
+# ---------------------------------------------------------------------------
+# adjoints
+
+def test_degrid_is_adjoint_of_wstack():
+    nx, ny, nw, n = 32, 24, 3, 400
+    u, v, w, vr, vi, _ = _random_case(21, n, nx, ny, nw)
+    vr = np.nan_to_num(vr).astype(np.float32)
+    wt = np.ones(n, np.float32)
+    st, gx = O.wstack(nw, u, v, w, vr, vi, wt, 1 / nx, 1 / nw, 7, nx, ny)
+    rng = np.random.default_rng(22)
+    y = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    st2, gy = O.degrid(y, u, v, w, wt, 1 / nx, 1 / nw, 7)
+    x = vr.astype(np.float64) + 1j * vi.astype(np.float64)
+    lhs = np.vdot(y, gx)   # <G x, y>
+    rhs = np.vdot(gy, x)   # <x, G^H y>
+    assert st == st2 == O.OK and abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_degrid_image_is_adjoint_of_image():
+    nx, ny, nw, n = 32, 32, 4, 300
+    u, v, w, vr, vi, _ = _random_case(31, n, nx, ny, nw, edge=False)
+    vr = np.nan_to_num(vr).astype(np.float32)
+    wt = np.ones(n, np.float32)
+    res, wmin, wmax = 1 / 64, -40.0, 55.0
+    ax = O.image(nw, u, v, w, vr, vi, wt, 7, nx, ny, res, wmin, wmax)
+    rng = np.random.default_rng(32)
+    y = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    ahy = O.degrid_image(y, nw, u, v, w, wt, 7, res, wmin, wmax)
+    x = vr.astype(np.float64) + 1j * vi.astype(np.float64)
+    assert abs(np.vdot(y, ax) - np.vdot(ahy, x)) < 1e-12 * abs(np.vdot(y, ax))
