@@ -204,7 +204,11 @@ def test_phase_dropin_matches_oracle(cuda, nx, ny, nw, wmin, wmax):
     re, im = re0.copy(), im0.copy()
     imager.phase_correction(cube.view(np.float64).ravel(), re, im, nx, ny, nw, nx, ny, res, wmin, wmax)
     want = O.phase_correction(cube, res, wmin, wmax, image=(re0 + 1j * im0).reshape(ny, nx))
-    assert rel((re + 1j * im).reshape(ny, nx), want) < 1e-12
+    got = (re + 1j * im).reshape(ny, nx)
+    # pixels with l^2+m^2 > 1 are NaN in the reference too (sqrt of a negative number)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert rel(got[ok], want[ok]) < 1e-12
     # device entry point, both precisions, overwrite semantics
     r, i = imager.phase(T(cube, cuda), res, wmin, wmax)
     want0 = O.phase_correction(cube, res, wmin, wmax)
