@@ -216,7 +216,8 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     const size_t off_counts = align256(off_status + 2 * sizeof(long long));
     const size_t off_offsets = align256(off_counts + (size_t)b.nb * sizeof(unsigned));
     const size_t off_partials = align256(off_offsets + (size_t)(b.nb + 1) * sizeof(long long));
-    const size_t bytes = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
+    const size_t off_slots = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
+    const size_t bytes = align256(off_slots + (size_t)(n > 0 ? n : 1) * 4 * sizeof(unsigned));
     int r = ensure(c, c.meta, bytes);
     if (r) return r;
     char* base = (char*)c.meta.ptr;
@@ -225,6 +226,7 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     b.counts = (unsigned*)(base + off_counts);
     b.offsets = (long long*)(base + off_offsets);
     b.partials = (long long*)(base + off_partials);
+    b.slots = (unsigned*)(base + off_slots);
     HIPCHK(stage(c, kStCount, st, [&] {
         hipError_t e = bucket_zero(b, st);
         if (e != hipSuccess) return e;
@@ -236,8 +238,6 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     const int err = (int)(c.status_host[1] & 0xffffffff);
     if (err) return fail(LOFAR_ECOORD);
     total = c.status_host[0];
-    // counts become the insertion cursors of the scatter
-    HIPCHK(bucket_zero(b, st));
     return LOFAR_OK;
 }
 

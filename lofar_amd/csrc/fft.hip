@@ -1,79 +1,141 @@
-// Batched per-plane 2D FFT for gfx950: Stockham autosort, radix-16 stages with
-// a radix-2/4/8 remainder, one or several lines per workgroup staged in LDS.
+// Batched per-plane 2D FFT for gfx950: Stockham autosort, radix-E stages
+// (E = 8 or 16 elements per thread) with a smaller remainder stage, lines
+// staged in LDS between stages.
 //
 // Semantics follow fftw_data() (fourier_transform.c:7-74): per w-plane an
 // unnormalised FFTW_BACKWARD transform over dims (n0=ny, n1=nx) of the [v][u]
 // plane, scaled by 1/(nx*ny).  Column transforms run first, then rows, so the
-// last pass (rows, contiguous) can fuse the w-phase screen and the plane sum of
+// last pass (rows, contiguous) fuses the w-phase screen and the plane sum of
 // phase_correction() (phase_correction.cu:151-210) and the image is written once.
+#include <cstdlib>
+
 #include "fft_kernels.h"
 
 namespace lofar {
 
+template <typename Real>
+using C2 = typename cplx<Real>::type;
+
 // ---------------------------------------------------------------------------
 // standalone passes
 
-// Row transforms: one line (row) per workgroup, N/E threads.
-template <typename Real, int N, int SIGN>
-__global__ __launch_bounds__(Plan<N>::TPL) void k_fft_rows(const typename cplx<Real>::type* in,
-                                                           typename cplx<Real>::type* out,
-                                                           const typename cplx<Real>::type* tw,
-                                                           Real scale)
+// Row transforms: one line (row) per workgroup.
+template <typename Real, int N, int E, int SIGN>
+__global__ __launch_bounds__((Plan<N, E>::TPL)) void k_fft_rows(const C2<Real>* in, C2<Real>* out,
+                                                              const C2<Real>* tw, Real scale)
 {
-    using T2 = typename cplx<Real>::type;
-    constexpr int E = Plan<N>::E;
-    __shared__ T2 lds[N];
+    using T2 = C2<Real>;
+    using P = Plan<N, E>;
+    __shared__ T2 lds[P::LDS];
     const long long line = blockIdx.x;
     const T2* src = in + line * N;
     T2* dst = out + line * N;
     const int t = threadIdx.x;
-    T2 v[E];
+    T2 v[P::E];
 #pragma unroll
-    for (int q = 0; q < E; q++) v[q] = src[Plan<N>::in_index(t, q)];
-    stages<T2, N, SIGN, 1>(v, lds, 1, t, tw);
+    for (int q = 0; q < P::E; q++) v[q] = src[P::in_index(t, q)];
+    stages<T2, N, E, SIGN, 1>(v, lds, 1, t, tw);
 #pragma unroll
-    for (int e = 0; e < E; e++) {
+    for (int e = 0; e < P::E; e++) {
         T2 x = v[e];
         x.x *= scale;
         x.y *= scale;
-        dst[Plan<N>::out_index(t, e)] = x;
+        dst[P::out_index(t, e)] = x;
     }
 }
 
-// Column transforms: W adjacent columns per workgroup (W*16 B or W*8 B contiguous
-// per row access), LDS laid out [N][W].  Workgroups are mapped so that the
-// strips one XCD runs at a time are adjacent in memory and share its L2.
-template <typename Real, int N, int W, int SIGN>
-__global__ __launch_bounds__(W* Plan<N>::TPL) void k_fft_cols(const typename cplx<Real>::type* in,
-                                                              typename cplx<Real>::type* out,
-                                                              const typename cplx<Real>::type* tw,
-                                                              int nx, int strips_per_plane,
-                                                              Real scale)
+// Column transforms: one strip of W adjacent columns per workgroup (W*16 or
+// W*8 bytes contiguous per row access), LDS laid out [slot][W].  Workgroups
+// are numbered so that the strips one XCD runs at a time are adjacent in
+// memory and share its L2 lines.
+template <typename Real, int N, int E, int W, int SIGN>
+__global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Real>* in,
+                                                                 C2<Real>* out,
+                                                                 const C2<Real>* tw, int nx,
+                                                                 int strips_per_plane,
+                                                                 long long nstrips, Real scale)
 {
-    using T2 = typename cplx<Real>::type;
-    constexpr int E = Plan<N>::E;
-    __shared__ T2 lds[N * W];
+    using T2 = C2<Real>;
+    using P = Plan<N, E>;
+    __shared__ T2 lds[P::LDS * W];
     const long long s = xcd_remap(blockIdx.x, gridDim.x);
     const long long plane = s / strips_per_plane;
     const int u0 = (int)(s % strips_per_plane) * W;
     const int c = threadIdx.x % W;
     const int t = threadIdx.x / W;
-    const bool active = u0 + c < nx;  // the last strip may be partial
+    const bool active = u0 + c < nx;  // the last strip of a plane may be partial
     const long long base = plane * (long long)N * nx + (active ? u0 + c : 0);
-    const T2* src = in + base;
-    T2* dst = out + base;
-    T2 v[E];
+    T2 v[P::E];
 #pragma unroll
-    for (int q = 0; q < E; q++)
-        v[q] = active ? src[(long long)Plan<N>::in_index(t, q) * nx] : T2{0, 0};
-    stages<T2, N, SIGN, 1>(v, lds + c, W, t, tw);
+    for (int q = 0; q < P::E; q++)
+        v[q] = active ? in[base + (long long)P::in_index(t, q) * nx] : T2{0, 0};
+    stages<T2, N, E, SIGN, 1>(v, lds + c, W, t, tw);
     if (active) {
 #pragma unroll
-        for (int e = 0; e < E; e++) {
+        for (int e = 0; e < P::E; e++) {
             T2 x = v[e];
             x.x *= scale;
             x.y *= scale;
-            dst[(long long)Plan<N>::out_index(t, e) * nx] = x;
+            out[base + (long long)P::out_index(t, e) * nx] = x;
+        }
+    }
+}
+
+// Persistent variant of k_fft_cols: each workgroup walks strips s, s+G, ...
+// and loads the next strip into registers while it transforms the current one.
+template <typename Real, int N, int E, int W, int SIGN>
+__global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols_pf(const C2<Real>* in,
+                                                                    C2<Real>* out,
+                                                                    const C2<Real>* tw, int nx,
+                                                                    int strips_per_plane,
+                                                                    long long nstrips, Real scale)
+{
+    using T2 = C2<Real>;
+    using P = Plan<N, E>;
+    __shared__ T2 lds[P::LDS * W];
+    const int c = threadIdx.x % W;
+    const int t = threadIdx.x / W;
+    const long long G = gridDim.x;
+    long long s = xcd_remap(blockIdx.x, G);
+    if (s >= nstrips) return;
+
+    auto base_of = [&](long long strip, bool& active) {
+        const long long plane = strip / strips_per_plane;
+        const int u0 = (int)(strip % strips_per_plane) * W;
+        active = u0 + c < nx;  // the last strip of a plane may be partial
+        return plane * (long long)N * nx + (active ? u0 + c : 0);
+    };
+    T2 v[P::E], nxt[P::E];
+    bool act;
+    long long b = base_of(s, act);
+#pragma unroll
+    for (int q = 0; q < P::E; q++)
+        v[q] = act ? in[b + (long long)P::in_index(t, q) * nx] : T2{0, 0};
+    for (; s < nstrips; s += G) {
+        const long long sn = s + G;
+        bool actn = false;
+        long long bn = 0;
+        if (sn < nstrips) {
+            bn = base_of(sn, actn);
+#pragma unroll
+            for (int q = 0; q < P::E; q++)
+                nxt[q] = actn ? in[bn + (long long)P::in_index(t, q) * nx] : T2{0, 0};
+        }
+        stages<T2, N, E, SIGN, 1>(v, lds + c, W, t, tw);
+        if (act) {
+#pragma unroll
+            for (int e = 0; e < P::E; e++) {
+                T2 x = v[e];
+                x.x *= scale;
+                x.y *= scale;
+                out[b + (long long)P::out_index(t, e) * nx] = x;
+            }
+        }
+        if (sn < nstrips) {
+#pragma unroll
+            for (int q = 0; q < P::E; q++) v[q] = nxt[q];
+            b = bn;
+            act = actn;
         }
     }
 }
@@ -81,120 +143,141 @@ __global__ __launch_bounds__(W* Plan<N>::TPL) void k_fft_cols(const typename cpl
 // ---------------------------------------------------------------------------
 // fused passes
 
-// Row transforms of every plane of one image row, each output multiplied by the
-// w-phase screen and summed over planes in registers (phase_correction.cu:151-210);
-// the image row is written once.  FP64 phase argument and FP64 accumulation in
-// both precision modes.
-template <typename Real, int N>
-__global__ __launch_bounds__(Plan<N>::TPL) void k_rows_phase(
-    const typename cplx<Real>::type* cube, const typename cplx<Real>::type* tw, int ny, int nw,
-    PhaseParams pp, double scale, double* img_re, double* img_im, int accumulate)
+// Per-pixel w-phase rotation for the plane recurrence: the factor of plane 0,
+// dwnorm*sqrt(1-r^2)*exp(i*phi_0), and the per-plane step exp(i*dphi) with
+// phi_w = 2*PI*(wmin + w*dw)*(sqrt(1-r^2)-1) (phase_correction.cu:160,182-183,
+// 193-195).  Without the phase (a single plane) both are 1.  FP64 throughout.
+__device__ __forceinline__ void phase_rotation(const PhaseParams& pp, int iu, double ycoord,
+                                               double2& z0, double2& step)
 {
-    using T2 = typename cplx<Real>::type;
-    constexpr int E = Plan<N>::E;
-    __shared__ T2 lds[N];
+    if (!pp.phase_on) {
+        z0 = double2{1.0, 0.0};
+        step = double2{1.0, 0.0};
+        return;
+    }
+    const double xcoord = pp.coord(iu, pp.xtot);
+    const double radius2 = (xcoord * xcoord + ycoord * ycoord);
+    const double fac = sqrt(1 - radius2);
+    const double n1 = fac - 1.0;
+    double s, c;
+    sincos(2.0 * LOFAR_REF_PI * pp.wmin * n1, &s, &c);
+    const double f = pp.dwnorm * fac;
+    z0 = double2{c * f, s * f};
+    sincos(2.0 * LOFAR_REF_PI * pp.dw * n1, &s, &c);
+    step = double2{c, s};
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Row transforms of every plane of one image row; each output is multiplied by
+// its w-phase factor and summed over planes in registers, and the image row is
+// written once.  The next plane's row is loaded while the current one is
+// transformed.  FP64 phase and accumulation in both precision modes.
+template <typename Real, int N, int E>
+__global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>* cube,
+                                                                const C2<Real>* tw, int ny, int nw,
+                                                                PhaseParams pp, double scale,
+                                                                double* img_re, double* img_im,
+                                                                int accumulate)
+{
+    using T2 = C2<Real>;
+    using P = Plan<N, E>;
+    __shared__ T2 lds[P::LDS];
     const int iv = blockIdx.x;
     const int t = threadIdx.x;
     const long long img_row = (long long)iv * N;
-
-    // per-pixel factor sqrt(1-l^2-m^2) for this thread's output pixels
-    const double ycoord = pp.coord(iv, pp.ytot);
-    double fac[E];
-    double sr[E], si[E];
+    const double ycoord = pp.phase_on ? pp.coord(iv, pp.ytot) : 0.0;
+    double2 z[P::E], stp[P::E], acc[P::E];
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int iu = Plan<N>::out_index(t, e);
-        const double xcoord = pp.coord(iu, pp.xtot);
-        const double radius2 = (xcoord * xcoord + ycoord * ycoord);
-        fac[e] = sqrt(1 - radius2);
-        sr[e] = 0.0;
-        si[e] = 0.0;
+    for (int e = 0; e < P::E; e++) {
+        phase_rotation(pp, P::out_index(t, e), ycoord, z[e], stp[e]);
+        acc[e] = double2{0.0, 0.0};
     }
+    T2 v[P::E], nxt[P::E];
+    const long long plane_stride = (long long)ny * N;
+    const T2* src = cube + img_row;
+#pragma unroll
+    for (int q = 0; q < P::E; q++) v[q] = src[P::in_index(t, q)];
     for (int iw = 0; iw < nw; iw++) {
-        const T2* src = cube + ((long long)iw * ny + iv) * N;
-        T2 v[E];
+        if (iw + 1 < nw) {
+            const T2* nsrc = src + (long long)(iw + 1) * plane_stride;
 #pragma unroll
-        for (int q = 0; q < E; q++) v[q] = src[Plan<N>::in_index(t, q)];
-        stages<T2, N, 1, 1>(v, lds, 1, t, tw);
-        const double wterm = pp.wmin + iw * pp.dw;
-        const double c2w = 2.0 * LOFAR_REF_PI * wterm;
+            for (int q = 0; q < P::E; q++) nxt[q] = nsrc[P::in_index(t, q)];
+        }
+        stages<T2, N, E, 1, 1>(v, lds, 1, t, tw);
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const double p = (double)v[e].x * scale;
-            const double q = (double)v[e].y * scale;
-            if (pp.phase_on) {
-                double s, r;
-                sincos(c2w * (fac[e] - 1.0), &s, &r);
-                sr[e] += (p * r - q * s) * pp.dwnorm * fac[e];
-                si[e] += (p * s + q * r) * pp.dwnorm * fac[e];
-            } else {
-                sr[e] += p;
-                si[e] += q;
-            }
+        for (int e = 0; e < P::E; e++) {
+            const double2 pq = double2{(double)v[e].x * scale, (double)v[e].y * scale};
+            const double2 term = cmul(pq, z[e]);
+            acc[e].x += term.x;
+            acc[e].y += term.y;
+            z[e] = cmul(z[e], stp[e]);
+        }
+        if (iw + 1 < nw) {
+#pragma unroll
+            for (int q = 0; q < P::E; q++) v[q] = nxt[q];
         }
     }
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const long long o = img_row + Plan<N>::out_index(t, e);
+    for (int e = 0; e < P::E; e++) {
+        const long long o = img_row + P::out_index(t, e);
         if (accumulate) {
-            img_re[o] += sr[e];
-            img_im[o] += si[e];
+            img_re[o] += acc[e].x;
+            img_im[o] += acc[e].y;
         } else {
-            img_re[o] = sr[e];
-            img_im[o] = si[e];
+            img_re[o] = acc[e].x;
+            img_im[o] = acc[e].y;
         }
     }
 }
 
-// Adjoint of k_rows_phase: for every plane, the image row times conj(phase_w)
-// times the same factor, forward row transform, plane row written out.
-template <typename Real, int N>
-__global__ __launch_bounds__(Plan<N>::TPL) void k_rows_phase_adj(
-    typename cplx<Real>::type* cube, const typename cplx<Real>::type* tw, int ny, int nw,
-    PhaseParams pp, double scale, const double* img_re, const double* img_im)
+// Adjoint of k_rows_phase: for every plane, the image row times the conjugate
+// w-phase factor, forward row transform, plane row written out.
+template <typename Real, int N, int E>
+__global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase_adj(C2<Real>* cube,
+                                                                    const C2<Real>* tw, int ny,
+                                                                    int nw, PhaseParams pp,
+                                                                    double scale,
+                                                                    const double* img_re,
+                                                                    const double* img_im)
 {
-    using T2 = typename cplx<Real>::type;
-    constexpr int E = Plan<N>::E;
-    __shared__ T2 lds[N];
+    using T2 = C2<Real>;
+    using P = Plan<N, E>;
+    __shared__ T2 lds[P::LDS];
     const int iv = blockIdx.x;
     const int t = threadIdx.x;
     const long long img_row = (long long)iv * N;
-    const double ycoord = pp.coord(iv, pp.ytot);
-    double fac[E], a[E], b[E];
+    const double ycoord = pp.phase_on ? pp.coord(iv, pp.ytot) : 0.0;
+    double2 z[P::E], stp[P::E], a[P::E];
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int iu = Plan<N>::in_index(t, q);
-        const double xcoord = pp.coord(iu, pp.xtot);
-        const double radius2 = (xcoord * xcoord + ycoord * ycoord);
-        fac[q] = sqrt(1 - radius2);
-        a[q] = img_re[img_row + iu];
-        b[q] = img_im[img_row + iu];
+    for (int q = 0; q < P::E; q++) {
+        const int iu = P::in_index(t, q);
+        phase_rotation(pp, iu, ycoord, z[q], stp[q]);
+        a[q] = double2{img_re[img_row + iu], img_im[img_row + iu]};
     }
+    const long long plane_stride = (long long)ny * N;
     for (int iw = 0; iw < nw; iw++) {
-        const double wterm = pp.wmin + iw * pp.dw;
-        const double c2w = 2.0 * LOFAR_REF_PI * wterm;
-        T2 v[E];
+        T2 v[P::E];
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-            double xr = a[q], xi = b[q];
-            if (pp.phase_on) {
-                double s, r;
-                sincos(c2w * (fac[q] - 1.0), &s, &r);
-                const double f = pp.dwnorm * fac[q];
-                xr = (a[q] * r + b[q] * s) * f;
-                xi = (b[q] * r - a[q] * s) * f;
-            }
+        for (int q = 0; q < P::E; q++) {
+            // a * conj(z)
+            const double xr = a[q].x * z[q].x + a[q].y * z[q].y;
+            const double xi = a[q].y * z[q].x - a[q].x * z[q].y;
             v[q].x = (Real)xr;
             v[q].y = (Real)xi;
+            z[q] = cmul(z[q], stp[q]);
         }
-        stages<T2, N, -1, 1>(v, lds, 1, t, tw);
-        T2* dst = cube + ((long long)iw * ny + iv) * N;
+        stages<T2, N, E, -1, 1>(v, lds, 1, t, tw);
+        T2* dst = cube + (long long)iw * plane_stride + img_row;
 #pragma unroll
-        for (int e = 0; e < E; e++) {
+        for (int e = 0; e < P::E; e++) {
             T2 x = v[e];
             x.x = (Real)(x.x * scale);
             x.y = (Real)(x.y * scale);
-            dst[Plan<N>::out_index(t, e)] = x;
+            dst[P::out_index(t, e)] = x;
         }
     }
 }
@@ -202,38 +285,81 @@ __global__ __launch_bounds__(Plan<N>::TPL) void k_rows_phase_adj(
 // ---------------------------------------------------------------------------
 // host-side dispatch
 
+// elements per thread of the fused row passes and of the column pass, and
+// columns per strip; LOFAR_FFT_CFG overrides the column configuration (tuning)
+constexpr int kRowE = 16;
+constexpr int kPhaseE = 8;
+
+static int col_cfg()
+{
+    static int cfg = -1;
+    if (cfg < 0) {
+        const char* s = getenv("LOFAR_FFT_CFG");
+        cfg = s ? atoi(s) : 0;
+    }
+    return cfg;
+}
+
+template <typename K>
+static int persistent_grid(K kernel, int threads, long long work)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    long long g = (long long)device_cus() * per_cu;
+    if (g > work) g = work;
+    return (int)(g < 1 ? 1 : g);
+}
+
 template <typename Real, int N>
 static hipError_t launch_rows_n(int sign, const void* in, void* out, const void* tw,
                                 long long lines, double scale, hipStream_t st)
 {
-    using T2 = typename cplx<Real>::type;
+    using T2 = C2<Real>;
+    constexpr int TPL = Plan<N, kRowE>::TPL;
     if (lines <= 0) return hipSuccess;
     if (sign > 0)
-        hipLaunchKernelGGL((k_fft_rows<Real, N, 1>), dim3((unsigned)lines), dim3(Plan<N>::TPL), 0,
+        hipLaunchKernelGGL((k_fft_rows<Real, N, kRowE, 1>), dim3((unsigned)lines), dim3(TPL), 0,
                            st, (const T2*)in, (T2*)out, (const T2*)tw, (Real)scale);
     else
-        hipLaunchKernelGGL((k_fft_rows<Real, N, -1>), dim3((unsigned)lines), dim3(Plan<N>::TPL), 0,
+        hipLaunchKernelGGL((k_fft_rows<Real, N, kRowE, -1>), dim3((unsigned)lines), dim3(TPL), 0,
                            st, (const T2*)in, (T2*)out, (const T2*)tw, (Real)scale);
     return hipGetLastError();
 }
 
-template <typename Real, int N>
-static hipError_t launch_cols_n(int sign, const void* in, void* out, const void* tw, int nx, int nw,
-                                double scale, hipStream_t st)
+template <typename Real, int N, int E, int WREQ, bool PERSISTENT>
+static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const void* tw, int nx,
+                                  int nw, double scale, hipStream_t st)
 {
-    using T2 = typename cplx<Real>::type;
-    constexpr int W = ColW<Real, N>::W;
+    using T2 = C2<Real>;
+    // at most 1024 threads per workgroup
+    constexpr int W = WREQ * Plan<N, E>::TPL <= 1024 ? WREQ : 1024 / Plan<N, E>::TPL;
+    constexpr int threads = W * Plan<N, E>::TPL;
     const int strips = (nx + W - 1) / W;
-    const long long blocks = (long long)strips * nw;
-    if (sign > 0)
-        hipLaunchKernelGGL((k_fft_cols<Real, N, W, 1>), dim3((unsigned)blocks),
-                           dim3(W * Plan<N>::TPL), 0, st, (const T2*)in, (T2*)out, (const T2*)tw,
-                           nx, strips, (Real)scale);
-    else
-        hipLaunchKernelGGL((k_fft_cols<Real, N, W, -1>), dim3((unsigned)blocks),
-                           dim3(W * Plan<N>::TPL), 0, st, (const T2*)in, (T2*)out, (const T2*)tw,
-                           nx, strips, (Real)scale);
+    const long long nstrips = (long long)strips * nw;
+    auto k = PERSISTENT ? (sign > 0 ? k_fft_cols_pf<Real, N, E, W, 1> : k_fft_cols_pf<Real, N, E, W, -1>)
+                        : (sign > 0 ? k_fft_cols<Real, N, E, W, 1> : k_fft_cols<Real, N, E, W, -1>);
+    const long long g = PERSISTENT ? persistent_grid(k, threads, nstrips) : nstrips;
+    hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(threads), 0, st, (const T2*)in, (T2*)out,
+                       (const T2*)tw, nx, strips, nstrips, (Real)scale);
     return hipGetLastError();
+}
+
+// Column-pass configurations by strip width (LDS per strip within 64 or 128 KiB);
+// entry 0 is the default.
+template <typename Real, int N>
+static hipError_t launch_cols_n(int sign, const void* in, void* out, const void* tw, int nx,
+                                int nw, double scale, hipStream_t st)
+{
+    constexpr int B = Plan<N, 16>::LDS * 2 * (int)sizeof(Real);
+    constexpr int W1 = 70000 / B >= 1 ? (70000 / B > 16 ? 16 : 70000 / B) : 1;     // ~64 KiB
+    constexpr int W2 = 140000 / B >= 1 ? (140000 / B > 16 ? 16 : 140000 / B) : 1;  // ~128 KiB
+    switch (col_cfg()) {
+        case 1: return launch_cols_cfg<Real, N, 16, W1, true>(sign, in, out, tw, nx, nw, scale, st);
+        case 2: return launch_cols_cfg<Real, N, 16, W1, false>(sign, in, out, tw, nx, nw, scale, st);
+        default: return launch_cols_cfg<Real, N, 16, W2, false>(sign, in, out, tw, nx, nw, scale, st);
+    }
 }
 
 template <typename Real, int N>
@@ -241,9 +367,9 @@ static hipError_t launch_rows_phase_n(const void* cube, const void* tw, int ny, 
                                       const PhaseParams& pp, double scale, double* re, double* im,
                                       int accumulate, hipStream_t st)
 {
-    using T2 = typename cplx<Real>::type;
-    hipLaunchKernelGGL((k_rows_phase<Real, N>), dim3(ny), dim3(Plan<N>::TPL), 0, st,
-                       (const T2*)cube, (const T2*)tw, ny, nw, pp, scale, re, im, accumulate);
+    using T2 = C2<Real>;
+    hipLaunchKernelGGL((k_rows_phase<Real, N, kPhaseE>), dim3(ny), dim3(Plan<N, kPhaseE>::TPL), 0,
+                       st, (const T2*)cube, (const T2*)tw, ny, nw, pp, scale, re, im, accumulate);
     return hipGetLastError();
 }
 
@@ -252,33 +378,34 @@ static hipError_t launch_rows_phase_adj_n(void* cube, const void* tw, int ny, in
                                           const PhaseParams& pp, double scale, const double* re,
                                           const double* im, hipStream_t st)
 {
-    using T2 = typename cplx<Real>::type;
-    hipLaunchKernelGGL((k_rows_phase_adj<Real, N>), dim3(ny), dim3(Plan<N>::TPL), 0, st, (T2*)cube,
-                       (const T2*)tw, ny, nw, pp, scale, re, im);
+    using T2 = C2<Real>;
+    hipLaunchKernelGGL((k_rows_phase_adj<Real, N, kPhaseE>), dim3(ny),
+                       dim3(Plan<N, kPhaseE>::TPL), 0, st, (T2*)cube, (const T2*)tw, ny, nw, pp,
+                       scale, re, im);
     return hipGetLastError();
 }
 
-#define LOFAR_FFT_SWITCH(N_, CALL)        \
-    switch (N_) {                         \
-        case 2: return CALL(2);           \
-        case 4: return CALL(4);           \
-        case 8: return CALL(8);           \
-        case 16: return CALL(16);         \
-        case 32: return CALL(32);         \
-        case 64: return CALL(64);         \
-        case 128: return CALL(128);       \
-        case 256: return CALL(256);       \
-        case 512: return CALL(512);       \
-        case 1024: return CALL(1024);     \
-        case 2048: return CALL(2048);     \
-        case 4096: return CALL(4096);     \
-        case 8192: return CALL(8192);     \
+#define LOFAR_FFT_SWITCH(N_, CALL)            \
+    switch (N_) {                             \
+        case 2: return CALL(2);               \
+        case 4: return CALL(4);               \
+        case 8: return CALL(8);               \
+        case 16: return CALL(16);             \
+        case 32: return CALL(32);             \
+        case 64: return CALL(64);             \
+        case 128: return CALL(128);           \
+        case 256: return CALL(256);           \
+        case 512: return CALL(512);           \
+        case 1024: return CALL(1024);         \
+        case 2048: return CALL(2048);         \
+        case 4096: return CALL(4096);         \
+        case 8192: return CALL(8192);         \
         default: return hipErrorInvalidValue; \
     }
 
 template <typename Real>
-hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx,
-                    long long lines, double scale, hipStream_t st)
+hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx, long long lines,
+                    double scale, hipStream_t st)
 {
 #define CALL_(n) launch_rows_n<Real, n>(sign, in, out, tw, lines, scale, st)
     LOFAR_FFT_SWITCH(nx, CALL_)
@@ -312,6 +439,19 @@ hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
 #define CALL_(n) launch_rows_phase_adj_n<Real, n>(cube, tw, ny, nw, pp, scale, re, im, st)
     LOFAR_FFT_SWITCH(nx, CALL_)
 #undef CALL_
+}
+
+int device_cus()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            cus = p.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
 }
 
 template hipError_t fft_rows<double>(int, const void*, void*, const void*, int, long long, double,

@@ -6,14 +6,15 @@ namespace lofar {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
-// Per-line plan: E elements per thread, TPL threads per line, radix-16 stages
-// and one radix-2/4/8 remainder stage (last).
-template <int N>
+// Per-line plan: E elements per thread, TPL threads per line, radix-E stages
+// and one smaller remainder stage (last) when log2(N) is not a multiple of log2(E).
+template <int N, int EREQ>
 struct Plan {
-    static constexpr int E = N < 16 ? N : 16;
+    static constexpr int E = N < EREQ ? N : EREQ;
     static constexpr int TPL = N / E;
     static constexpr int LOG = ilog2(N);
-    static constexpr int RL = (N < 16) ? N : ((LOG % 4 == 0) ? 16 : (1 << (LOG % 4)));
+    static constexpr int LOGE = ilog2(E);
+    static constexpr int RL = (N <= E) ? N : ((LOG % LOGE == 0) ? E : (1 << (LOG % LOGE)));
     // element of the line thread t loads into v[q] before the first stage
     __device__ static __forceinline__ int in_index(int t, int q) { return t + q * TPL; }
     // element of the line thread t holds in v[e] after the last stage
@@ -21,15 +22,11 @@ struct Plan {
     {
         return t + (e / RL) * TPL + (e % RL) * (N / RL);
     }
-};
-
-// Columns per workgroup of the column pass: about 128 KiB of LDS, at most 1024 threads.
-template <typename Real, int N>
-struct ColW {
-    static constexpr int bytes = N * 2 * (int)sizeof(Real);
-    static constexpr int w0 = 131072 / bytes < 1 ? 1 : (131072 / bytes > 16 ? 16 : 131072 / bytes);
-    static constexpr int wmax = 1024 / Plan<N>::TPL;
-    static constexpr int W = w0 < wmax ? w0 : wmax;
+    // LDS slot of element i: one pad slot every E elements, so the first
+    // exchange (thread b writes b*E..b*E+E-1) does not put every lane of a
+    // wave on the same banks
+    __device__ static __forceinline__ int pad(int i) { return i + (i >> LOGE); }
+    static constexpr int LDS = N + N / E;  // slots per line
 };
 
 // Spread consecutive workgroups of one XCD (blockIdx % 8) over a contiguous range.
@@ -188,15 +185,16 @@ __device__ __forceinline__ void dft(T2* v)
 
 // Stockham stages from span NS on.  On entry v[m*R+q] holds element b+q*(N/R)
 // of butterfly b = t + m*TPL; on return v holds the outputs in the order of
-// Plan<N>::out_index.  `lds` addresses element i of this line at lds[i*ls].
+// Plan<N,E>::out_index.  `lds` addresses element i of this line at
+// lds[Plan<N,E>::pad(i)*ls] (Plan<N,E>::LDS slots per line).
 // Twiddles: tw[m] = exp(+2*pi*i*m/N), conjugated for SIGN < 0.
-template <typename T2, int N, int SIGN, int NS>
-__device__ __forceinline__ void stages(T2 (&v)[Plan<N>::E], T2* lds, int ls, int t,
-                                       const T2* __restrict__ tw)
+template <typename T2, int N, int EREQ, int SIGN, int NS>
+__device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* __restrict__ tw)
 {
-    constexpr int E = Plan<N>::E;
-    constexpr int TPL = Plan<N>::TPL;
-    constexpr int R = (N / NS >= 16) ? 16 : N / NS;
+    using P = Plan<N, EREQ>;
+    constexpr int E = P::E;
+    constexpr int TPL = P::TPL;
+    constexpr int R = (N / NS >= E) ? E : N / NS;
     constexpr int M = E / R;
 #pragma unroll
     for (int m = 0; m < M; m++) {
@@ -211,7 +209,7 @@ __device__ __forceinline__ void stages(T2 (&v)[Plan<N>::E], T2* lds, int ls, int
     }
     if constexpr (NS * R < N) {
         constexpr int NS2 = NS * R;
-        constexpr int R2 = (N / NS2 >= 16) ? 16 : N / NS2;
+        constexpr int R2 = (N / NS2 >= E) ? E : N / NS2;
         constexpr int M2 = E / R2;
         __syncthreads();
 #pragma unroll
@@ -220,16 +218,16 @@ __device__ __forceinline__ void stages(T2 (&v)[Plan<N>::E], T2* lds, int ls, int
             const int k = b & (NS - 1);
             const int base = (b - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) lds[(base + q * NS) * ls] = v[m * R + q];
+            for (int q = 0; q < R; q++) lds[P::pad(base + q * NS) * ls] = v[m * R + q];
         }
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < M2; m++) {
             const int b = t + m * TPL;
 #pragma unroll
-            for (int q = 0; q < R2; q++) v[m * R2 + q] = lds[(b + q * (N / R2)) * ls];
+            for (int q = 0; q < R2; q++) v[m * R2 + q] = lds[P::pad(b + q * (N / R2)) * ls];
         }
-        stages<T2, N, SIGN, NS2>(v, lds, ls, t, tw);
+        stages<T2, N, EREQ, SIGN, NS2>(v, lds, ls, t, tw);
     }
 }
 
@@ -251,5 +249,8 @@ template <typename Real>
 hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
                           const PhaseParams& pp, double scale, const double* re, const double* im,
                           hipStream_t st);
+
+// number of compute units of the current device (cached; persistent-grid sizing)
+int device_cus();
 
 }  // namespace lofar

@@ -8,6 +8,8 @@
 // window clipped to that tile, so no two workgroups write the same cell and no
 // global atomics or zero-fill pass are needed.  The uv-tile buckets are built
 // by a counting sort: count -> exclusive scan -> scatter.
+#include <cstdlib>
+
 #include "grid_kernels.h"
 
 namespace lofar {
@@ -23,9 +25,12 @@ __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, long long n, unsi
     if (i < n) p[i] = v;
 }
 
+// Counts every (sample, tile) pair and keeps the position the counting atomic
+// returned, so the scatter places entries without atomics.  A window spans at
+// most 2x2 tiles (2K+1 <= 17 < kTile): slot t = 2*(tv-tv0) + (tu-tu0).
 __global__ __launch_bounds__(256) void k_count(int mode, long long n, const double* uu,
                                                const double* vv, const double* ww, Geometry g,
-                                               unsigned* counts, int* err)
+                                               unsigned* counts, unsigned* slots, int* err)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -37,13 +42,15 @@ __global__ __launch_bounds__(256) void k_count(int mode, long long n, const doub
     const long long pbase = (long long)s.gw * g.ntv;
     if (mode == kBucketHome) {
         const int cu = min(max(s.gu, 0), g.nx - 1), cv = min(max(s.gv, 0), g.ny - 1);
-        atomicAdd(&counts[(pbase + cv / kTile) * g.ntu + cu / kTile], 1u);
+        slots[4 * i] = atomicAdd(&counts[(pbase + cv / kTile) * g.ntu + cu / kTile], 1u);
         return;
     }
     if (s.j0 > s.j1 || s.k0 > s.k1) return;
-    for (int tv = s.k0 / kTile; tv <= s.k1 / kTile; tv++)
-        for (int tu = s.j0 / kTile; tu <= s.j1 / kTile; tu++)
-            atomicAdd(&counts[(pbase + tv) * g.ntu + tu], 1u);
+    const int tv0 = s.k0 / kTile, tu0 = s.j0 / kTile;
+    for (int tv = tv0; tv <= s.k1 / kTile; tv++)
+        for (int tu = tu0; tu <= s.j1 / kTile; tu++)
+            slots[4 * i + 2 * (tv - tv0) + (tu - tu0)] =
+                atomicAdd(&counts[(pbase + tv) * g.ntu + tu], 1u);
 }
 
 __global__ __launch_bounds__(256) void k_scan_local(const unsigned* in, long long* out,
@@ -142,8 +149,9 @@ __global__ __launch_bounds__(256) void k_scatter_grid(long long n, long long F, 
                                                       const double* uu, const double* vv,
                                                       const double* ww, const float* vr,
                                                       const float* vi, const float* wt, Geometry g,
-                                                      const long long* offsets, unsigned* cursor,
-                                                      void* entries, uint32_t* wins)
+                                                      const long long* offsets,
+                                                      const unsigned* slots, void* entries,
+                                                      uint32_t* wins)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -153,14 +161,15 @@ __global__ __launch_bounds__(256) void k_scatter_grid(long long n, long long F, 
     double sr, si;
     sample_vis(i, F, P, vr, vi, wt, sr, si);
     const long long pbase = (long long)s.gw * g.ntv;
-    for (int tv = s.k0 / kTile; tv <= s.k1 / kTile; tv++) {
+    const int tv0 = s.k0 / kTile, tu0 = s.j0 / kTile;
+    for (int tv = tv0; tv <= s.k1 / kTile; tv++) {
         const int v0 = tv * kTile;
         const int b0 = max(s.k0, v0) - v0, b1 = min(s.k1, v0 + kTile - 1) - v0;
-        for (int tu = s.j0 / kTile; tu <= s.j1 / kTile; tu++) {
+        for (int tu = tu0; tu <= s.j1 / kTile; tu++) {
             const int u0 = tu * kTile;
             const int a0 = max(s.j0, u0) - u0, a1 = min(s.j1, u0 + kTile - 1) - u0;
             const long long key = (pbase + tv) * g.ntu + tu;
-            const long long pos = offsets[key] + atomicAdd(&cursor[key], 1u);
+            const long long pos = offsets[key] + slots[4 * i + 2 * (tv - tv0) + (tu - tu0)];
             wins[pos] = pack_win(a0, a1, b0, b1);
             if constexpr (FP64) {
                 Entry64 e;
@@ -184,7 +193,7 @@ __global__ __launch_bounds__(256) void k_scatter_grid(long long n, long long F, 
 __global__ __launch_bounds__(256) void k_scatter_home(long long n, const double* uu,
                                                       const double* vv, const double* ww,
                                                       Geometry g, const long long* offsets,
-                                                      unsigned* cursor, DEntry* entries)
+                                                      const unsigned* slots, DEntry* entries)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(256) void k_scatter_home(long long n, const double*
     if (!sample_window(uu[i], vv[i], ww[i], g, s)) return;
     const int cu = min(max(s.gu, 0), g.nx - 1), cv = min(max(s.gv, 0), g.ny - 1);
     const long long key = ((long long)s.gw * g.ntv + cv / kTile) * g.ntu + cu / kTile;
-    const long long pos = offsets[key] + atomicAdd(&cursor[key], 1u);
+    const long long pos = offsets[key] + slots[4 * i];
     DEntry e;
     e.pu = s.pu;
     e.pv = s.pv;
@@ -329,6 +338,123 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, const long long*
     }
 }
 
+// Same tile ownership, but WIN lanes share one entry: lane dj of a group owns
+// window column dj, evaluates exp(-du^2/2) for it and exp(-dv^2/2) for window
+// row dj, and walks the rows taking each row's factor from the group lane that
+// computed it.  The atomics of one instruction then hit runs of WIN adjacent
+// cells (few LDS bank conflicts) instead of 64 scattered ones.
+template <int WIN, bool FP64>
+__global__ __launch_bounds__(256) void k_grid_tiles_cols(Geometry g, const long long* offsets,
+                                                         const void* entries,
+                                                         const uint32_t* wins, void* cube,
+                                                         int accumulate)
+{
+    constexpr int GPW = 64 / WIN;  // entries per wave per step
+    __shared__ double acc[2 * kTile * kTile];
+    const long long bkt = blockIdx.x;
+    const int tu = (int)(bkt % g.ntu);
+    const long long r = bkt / g.ntu;
+    const int tv = (int)(r % g.ntv);
+    const long long w = r / g.ntv;
+    const int u0 = tu * kTile, v0 = tv * kTile;
+    const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
+    const long long beg = offsets[bkt], end = offsets[bkt + 1];
+    const long long cell0 = (w * g.ny + v0) * (long long)g.nx + u0;
+    const int tid = threadIdx.x;
+
+    if (beg == end) {
+        if (!accumulate) {
+            for (int c = tid; c < kTile * kTile; c += 256) {
+                const int row = c / kTile, col = c % kTile;
+                if (row < trows && col < tcols) {
+                    const long long o = cell0 + (long long)row * g.nx + col;
+                    if constexpr (FP64)
+                        ((double2*)cube)[o] = double2{0.0, 0.0};
+                    else
+                        ((float2*)cube)[o] = float2{0.0f, 0.0f};
+                }
+            }
+        }
+        return;
+    }
+    for (int c = tid; c < kTile * kTile; c += 256) {
+        const int row = c / kTile, col = c % kTile;
+        double re = 0.0, im = 0.0;
+        if (accumulate && row < trows && col < tcols) {
+            const long long o = cell0 + (long long)row * g.nx + col;
+            if constexpr (FP64) {
+                const double2 x = ((const double2*)cube)[o];
+                re = x.x;
+                im = x.y;
+            } else {
+                const float2 x = ((const float2*)cube)[o];
+                re = x.x;
+                im = x.y;
+            }
+        }
+        acc[2 * c] = re;
+        acc[2 * c + 1] = im;
+    }
+    __syncthreads();
+
+    const double norm = 0.5 / LOFAR_REF_PI;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int grp = lane / WIN, dj = lane % WIN;
+    const int src0 = (grp < GPW ? grp : 0) * WIN;  // first lane of this group
+    for (long long e0 = beg + (long long)wave * GPW; e0 < end; e0 += 4 * GPW) {
+        const long long e = e0 + grp;
+        const bool live = grp < GPW && e < end;
+        const long long ee = live ? e : beg;  // dead lanes read a valid entry and add nothing
+        const uint32_t wn = wins[ee];
+        const int a0 = wn & 255, na = (int)((wn >> 8) & 255) - a0;
+        const int b0 = (wn >> 16) & 255, nb = (int)(wn >> 24) - b0;
+        const bool col_ok = live && dj <= na;
+        double* colp = &acc[2 * (b0 * kTile + a0 + dj)];
+        if constexpr (FP64) {
+            const Entry64 en = ((const Entry64*)entries)[ee];
+            const double du = (double)(u0 + a0 + dj) + 0.5 - en.pu;
+            const double eu = exp(-(du * du) * 0.5);
+            const double dvm = (double)(v0 + b0 + dj) + 0.5 - en.pv;
+            const double evm = norm * exp(-(dvm * dvm) * 0.5);
+            const double fr = eu * en.vr, fi = eu * en.vi;
+#pragma unroll
+            for (int kq = 0; kq < WIN; kq++) {
+                const double ev = __shfl(evm, src0 + kq);
+                if (col_ok && kq <= nb) {
+                    atomicAdd(colp + 2 * kq * kTile, ev * fr);
+                    atomicAdd(colp + 2 * kq * kTile + 1, ev * fi);
+                }
+            }
+        } else {
+            const Entry32 en = ((const Entry32*)entries)[ee];
+            const float du = en.du0 + (float)dj;
+            const float eu = expf(-(du * du) * 0.5f);
+            const float dvm = en.dv0 + (float)dj;
+            const float evm = (float)norm * expf(-(dvm * dvm) * 0.5f);
+            const float fr = eu * en.vr, fi = eu * en.vi;
+#pragma unroll
+            for (int kq = 0; kq < WIN; kq++) {
+                const float ev = __shfl(evm, src0 + kq);
+                if (col_ok && kq <= nb) {
+                    atomicAdd(colp + 2 * kq * kTile, (double)(ev * fr));
+                    atomicAdd(colp + 2 * kq * kTile + 1, (double)(ev * fi));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int c = tid; c < kTile * kTile; c += 256) {
+        const int row = c / kTile, col = c % kTile;
+        if (row < trows && col < tcols) {
+            const long long o = cell0 + (long long)row * g.nx + col;
+            if constexpr (FP64)
+                ((double2*)cube)[o] = double2{acc[2 * c], acc[2 * c + 1]};
+            else
+                ((float2*)cube)[o] = float2{(float)acc[2 * c], (float)acc[2 * c + 1]};
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // degridding (adjoint of gridding)
 
@@ -439,7 +565,7 @@ hipError_t bucket_count(int mode, long long n, const double* uu, const double* v
 {
     if (n > 0)
         hipLaunchKernelGGL(k_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, mode, n, uu, vv,
-                           ww, g, b.counts, b.err);
+                           ww, g, b.counts, b.slots, b.err);
     return hipGetLastError();
 }
 
@@ -464,10 +590,10 @@ hipError_t bucket_scatter_grid(int precision, long long n, long long F, long lon
     if (n <= 0) return hipSuccess;
     if (precision == LOFAR_FP64)
         hipLaunchKernelGGL((k_scatter_grid<true>), dim3(blocks_for(n, 256)), dim3(256), 0, st, n, F,
-                           P, uu, vv, ww, vr, vi, wt, g, b.offsets, b.counts, entries, wins);
+                           P, uu, vv, ww, vr, vi, wt, g, b.offsets, b.slots, entries, wins);
     else
         hipLaunchKernelGGL((k_scatter_grid<false>), dim3(blocks_for(n, 256)), dim3(256), 0, st, n,
-                           F, P, uu, vv, ww, vr, vi, wt, g, b.offsets, b.counts, entries, wins);
+                           F, P, uu, vv, ww, vr, vi, wt, g, b.offsets, b.slots, entries, wins);
     return hipGetLastError();
 }
 
@@ -477,20 +603,42 @@ hipError_t bucket_scatter_home(long long n, const double* uu, const double* vv, 
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter_home, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, uu, vv, ww, g,
-                       b.offsets, b.counts, entries);
+                       b.offsets, b.slots, entries);
     return hipGetLastError();
+}
+
+// gridding kernel variant: 0 = one lane per entry, 1 = WIN lanes per entry
+// (LOFAR_GRID_CFG overrides, for tuning)
+static int grid_cfg()
+{
+    static int cfg = -1;
+    if (cfg < 0) {
+        const char* s = getenv("LOFAR_GRID_CFG");
+        cfg = s ? atoi(s) : 0;
+    }
+    return cfg;
 }
 
 template <int WIN>
 static void launch_grid(int precision, const Geometry& g, const Buckets& b, const void* entries,
                         const uint32_t* wins, void* cube, int accumulate, hipStream_t st)
 {
-    if (precision == LOFAR_FP64)
-        hipLaunchKernelGGL((k_grid_tiles<WIN, true>), dim3((unsigned)b.nb), dim3(256), 0, st, g,
-                           b.offsets, entries, wins, cube, accumulate);
-    else
-        hipLaunchKernelGGL((k_grid_tiles<WIN, false>), dim3((unsigned)b.nb), dim3(256), 0, st, g,
-                           b.offsets, entries, wins, cube, accumulate);
+    const bool cols = grid_cfg() == 1;
+    if (precision == LOFAR_FP64) {
+        if (cols)
+            hipLaunchKernelGGL((k_grid_tiles_cols<WIN, true>), dim3((unsigned)b.nb), dim3(256), 0,
+                               st, g, b.offsets, entries, wins, cube, accumulate);
+        else
+            hipLaunchKernelGGL((k_grid_tiles<WIN, true>), dim3((unsigned)b.nb), dim3(256), 0, st,
+                               g, b.offsets, entries, wins, cube, accumulate);
+    } else {
+        if (cols)
+            hipLaunchKernelGGL((k_grid_tiles_cols<WIN, false>), dim3((unsigned)b.nb), dim3(256), 0,
+                               st, g, b.offsets, entries, wins, cube, accumulate);
+        else
+            hipLaunchKernelGGL((k_grid_tiles<WIN, false>), dim3((unsigned)b.nb), dim3(256), 0, st,
+                               g, b.offsets, entries, wins, cube, accumulate);
+    }
 }
 
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,

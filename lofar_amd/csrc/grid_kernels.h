@@ -34,7 +34,8 @@ __device__ __forceinline__ bool sample_window(double u, double v, double w, cons
 // Workspace layout for one bucketing of n samples into nb tiles.
 struct Buckets {
     long long nb;          // number of (w, tile_v, tile_u) buckets
-    unsigned* counts;      // [nb], reused as insertion cursors
+    unsigned* counts;      // [nb]
+    unsigned* slots;       // [4n]: position of each (sample, tile) pair within its bucket
     long long* offsets;    // [nb+1]
     long long* partials;   // scan partials
     long long* total;      // host-visible total entry count (status[0])
