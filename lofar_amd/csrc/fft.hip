@@ -48,7 +48,7 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_fft_rows(const C2<Real>* 
 // W*8 bytes contiguous per row access), LDS laid out [slot][W].  Workgroups
 // are numbered so that the strips one XCD runs at a time are adjacent in
 // memory and share its L2 lines.
-template <typename Real, int N, int E, int W, int SIGN>
+template <typename Real, int N, int E, int W, int ROUNDS, int SIGN>
 __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Real>* in,
                                                                  C2<Real>* out,
                                                                  const C2<Real>* tw, int nx,
@@ -57,7 +57,8 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
-    __shared__ T2 lds[P::LDS * W];
+    constexpr int WR = W / ROUNDS;  // columns sharing the LDS buffer in one round
+    __shared__ T2 lds[P::LDS * WR];
     const long long s = xcd_remap(blockIdx.x, gridDim.x);
     const long long plane = s / strips_per_plane;
     const int u0 = (int)(s % strips_per_plane) * W;
@@ -69,7 +70,7 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
 #pragma unroll
     for (int q = 0; q < P::E; q++)
         v[q] = active ? in[base + (long long)P::in_index(t, q) * nx] : T2{0, 0};
-    stages<T2, N, E, SIGN, 1>(v, lds + c, W, t, tw);
+    stages<T2, N, E, SIGN, 1, ROUNDS>(v, lds + c % WR, WR, t, tw, c / WR);
     if (active) {
 #pragma unroll
         for (int e = 0; e < P::E; e++) {
@@ -81,90 +82,16 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
     }
 }
 
-// Persistent variant of k_fft_cols: each workgroup walks strips s, s+G, ...
-// and loads the next strip into registers while it transforms the current one.
-template <typename Real, int N, int E, int W, int SIGN>
-__global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols_pf(const C2<Real>* in,
-                                                                    C2<Real>* out,
-                                                                    const C2<Real>* tw, int nx,
-                                                                    int strips_per_plane,
-                                                                    long long nstrips, Real scale)
-{
-    using T2 = C2<Real>;
-    using P = Plan<N, E>;
-    __shared__ T2 lds[P::LDS * W];
-    const int c = threadIdx.x % W;
-    const int t = threadIdx.x / W;
-    const long long G = gridDim.x;
-    long long s = xcd_remap(blockIdx.x, G);
-    if (s >= nstrips) return;
-
-    auto base_of = [&](long long strip, bool& active) {
-        const long long plane = strip / strips_per_plane;
-        const int u0 = (int)(strip % strips_per_plane) * W;
-        active = u0 + c < nx;  // the last strip of a plane may be partial
-        return plane * (long long)N * nx + (active ? u0 + c : 0);
-    };
-    T2 v[P::E], nxt[P::E];
-    bool act;
-    long long b = base_of(s, act);
-#pragma unroll
-    for (int q = 0; q < P::E; q++)
-        v[q] = act ? in[b + (long long)P::in_index(t, q) * nx] : T2{0, 0};
-    for (; s < nstrips; s += G) {
-        const long long sn = s + G;
-        bool actn = false;
-        long long bn = 0;
-        if (sn < nstrips) {
-            bn = base_of(sn, actn);
-#pragma unroll
-            for (int q = 0; q < P::E; q++)
-                nxt[q] = actn ? in[bn + (long long)P::in_index(t, q) * nx] : T2{0, 0};
-        }
-        stages<T2, N, E, SIGN, 1>(v, lds + c, W, t, tw);
-        if (act) {
-#pragma unroll
-            for (int e = 0; e < P::E; e++) {
-                T2 x = v[e];
-                x.x *= scale;
-                x.y *= scale;
-                out[b + (long long)P::out_index(t, e) * nx] = x;
-            }
-        }
-        if (sn < nstrips) {
-#pragma unroll
-            for (int q = 0; q < P::E; q++) v[q] = nxt[q];
-            b = bn;
-            act = actn;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // fused passes
 
-// Per-pixel w-phase rotation for the plane recurrence: the factor of plane 0,
-// dwnorm*sqrt(1-r^2)*exp(i*phi_0), and the per-plane step exp(i*dphi) with
-// phi_w = 2*PI*(wmin + w*dw)*(sqrt(1-r^2)-1) (phase_correction.cu:160,182-183,
-// 193-195).  Without the phase (a single plane) both are 1.  FP64 throughout.
-__device__ __forceinline__ void phase_rotation(const PhaseParams& pp, int iu, double ycoord,
-                                               double2& z0, double2& step)
+// Per-pixel factor sqrt(1-l^2-m^2) (phase_correction.cu:164-172); 1 without the phase.
+__device__ __forceinline__ double pixel_fac(const PhaseParams& pp, int iu, double ycoord)
 {
-    if (!pp.phase_on) {
-        z0 = double2{1.0, 0.0};
-        step = double2{1.0, 0.0};
-        return;
-    }
+    if (!pp.phase_on) return 1.0;
     const double xcoord = pp.coord(iu, pp.xtot);
     const double radius2 = (xcoord * xcoord + ycoord * ycoord);
-    const double fac = sqrt(1 - radius2);
-    const double n1 = fac - 1.0;
-    double s, c;
-    sincos(2.0 * LOFAR_REF_PI * pp.wmin * n1, &s, &c);
-    const double f = pp.dwnorm * fac;
-    z0 = double2{c * f, s * f};
-    sincos(2.0 * LOFAR_REF_PI * pp.dw * n1, &s, &c);
-    step = double2{c, s};
+    return sqrt(1 - radius2);
 }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
@@ -172,11 +99,71 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
     return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
+// w-phase factor of one pixel for the plane recurrence: z0 = dwnorm*fac*exp(i*phi_0)
+// and step = exp(i*(phi_1-phi_0)), with phi_w = 2*PI*(wmin + w*dw)*(fac-1)
+// (phase_correction.cu:160,182-183,193-195), so plane w's factor is z0*step^w.
+// Without the phase (a single plane) both are 1.  FP64 throughout.
+__device__ __forceinline__ void phase_rotation(const PhaseParams& pp, double fac, double2& z0,
+                                               double2& step)
+{
+    if (!pp.phase_on) {
+        z0 = double2{1.0, 0.0};
+        step = double2{1.0, 0.0};
+        return;
+    }
+    const double n1 = fac - 1.0;
+    double sn, cs;
+    sincos_w(2.0 * LOFAR_REF_PI * pp.wmin * n1, &sn, &cs);
+    const double f = pp.dwnorm * fac;
+    z0 = double2{cs * f, sn * f};
+    sincos_w(2.0 * LOFAR_REF_PI * pp.dw * n1, &sn, &cs);
+    step = double2{cs, sn};
+}
+
+// Per-pixel phase state of the fused row passes.  RECUR: plane factors by the
+// complex recurrence z_{w+1} = z_w*step (two complex numbers per pixel);
+// otherwise each plane's factor is evaluated directly with sincos_w (one double
+// per pixel).
+template <bool RECUR>
+struct PixelPhase;
+
+template <>
+struct PixelPhase<true> {
+    double2 z, stp;
+    __device__ __forceinline__ void init(const PhaseParams& pp, double fac)
+    {
+        phase_rotation(pp, fac, z, stp);
+    }
+    __device__ __forceinline__ double2 next(const PhaseParams&, double)
+    {
+        const double2 f = z;
+        z = cmul(z, stp);
+        return f;
+    }
+};
+
+template <>
+struct PixelPhase<false> {
+    double fac;
+    __device__ __forceinline__ void init(const PhaseParams&, double f) { fac = f; }
+    // factor of the plane whose 2*PI*wterm is c2w
+    __device__ __forceinline__ double2 next(const PhaseParams& pp, double c2w)
+    {
+        if (!pp.phase_on) return double2{1.0, 0.0};
+        double sn, cs;
+        sincos_w(c2w * (fac - 1.0), &sn, &cs);
+        const double f = pp.dwnorm * fac;
+        return double2{cs * f, sn * f};
+    }
+};
+
 // Row transforms of every plane of one image row; each output is multiplied by
-// its w-phase factor and summed over planes in registers, and the image row is
-// written once.  The next plane's row is loaded while the current one is
-// transformed.  FP64 phase and accumulation in both precision modes.
-template <typename Real, int N, int E>
+// its w-phase factor exp(i*phi_w)*dwnorm*sqrt(1-r^2) (phase_correction.cu:160,
+// 182-183,193-195) and summed over planes in registers, and the image row is
+// written once.  The thread's twiddle factors are loaded into registers once
+// (TWREG) since every plane's row uses the same ones.  FP64 phase and
+// accumulation in both precision modes.
+template <typename Real, int N, int E, bool RECUR, bool TWREG>
 __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>* cube,
                                                                 const C2<Real>* tw, int ny, int nw,
                                                                 PhaseParams pp, double scale,
@@ -185,40 +172,39 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
+    constexpr int NTW = tw_count<N, E>(1);
     __shared__ T2 lds[P::LDS];
     const int iv = blockIdx.x;
     const int t = threadIdx.x;
     const long long img_row = (long long)iv * N;
     const double ycoord = pp.phase_on ? pp.coord(iv, pp.ytot) : 0.0;
-    double2 z[P::E], stp[P::E], acc[P::E];
+    PixelPhase<RECUR> ph[P::E];
+    double2 acc[P::E];
 #pragma unroll
     for (int e = 0; e < P::E; e++) {
-        phase_rotation(pp, P::out_index(t, e), ycoord, z[e], stp[e]);
+        ph[e].init(pp, pixel_fac(pp, P::out_index(t, e), ycoord));
         acc[e] = double2{0.0, 0.0};
     }
-    T2 v[P::E], nxt[P::E];
+    T2 twr[NTW > 0 ? NTW : 1];
+    if constexpr (TWREG && NTW > 0) load_twiddles<T2, N, E, 1>(twr, t, tw);
     const long long plane_stride = (long long)ny * N;
     const T2* src = cube + img_row;
-#pragma unroll
-    for (int q = 0; q < P::E; q++) v[q] = src[P::in_index(t, q)];
+    T2 v[P::E];
     for (int iw = 0; iw < nw; iw++) {
-        if (iw + 1 < nw) {
-            const T2* nsrc = src + (long long)(iw + 1) * plane_stride;
+        const T2* psrc = src + (long long)iw * plane_stride;
 #pragma unroll
-            for (int q = 0; q < P::E; q++) nxt[q] = nsrc[P::in_index(t, q)];
-        }
-        stages<T2, N, E, 1, 1>(v, lds, 1, t, tw);
+        for (int q = 0; q < P::E; q++) v[q] = psrc[P::in_index(t, q)];
+        if constexpr (TWREG)
+            stages_rt<T2, N, E, 1, 1>(v, lds, t, twr);
+        else
+            stages<T2, N, E, 1, 1>(v, lds, 1, t, tw);
+        const double c2w = 2.0 * LOFAR_REF_PI * (pp.wmin + iw * pp.dw);
 #pragma unroll
         for (int e = 0; e < P::E; e++) {
             const double2 pq = double2{(double)v[e].x * scale, (double)v[e].y * scale};
-            const double2 term = cmul(pq, z[e]);
+            const double2 term = cmul(pq, ph[e].next(pp, c2w));
             acc[e].x += term.x;
             acc[e].y += term.y;
-            z[e] = cmul(z[e], stp[e]);
-        }
-        if (iw + 1 < nw) {
-#pragma unroll
-            for (int q = 0; q < P::E; q++) v[q] = nxt[q];
         }
     }
 #pragma unroll
@@ -236,7 +222,7 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
 
 // Adjoint of k_rows_phase: for every plane, the image row times the conjugate
 // w-phase factor, forward row transform, plane row written out.
-template <typename Real, int N, int E>
+template <typename Real, int N, int E, bool RECUR, bool TWREG>
 __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase_adj(C2<Real>* cube,
                                                                     const C2<Real>* tw, int ny,
                                                                     int nw, PhaseParams pp,
@@ -246,31 +232,37 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase_adj(C2<Real>* 
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
+    constexpr int NTW = tw_count<N, E>(1);
     __shared__ T2 lds[P::LDS];
     const int iv = blockIdx.x;
     const int t = threadIdx.x;
     const long long img_row = (long long)iv * N;
     const double ycoord = pp.phase_on ? pp.coord(iv, pp.ytot) : 0.0;
-    double2 z[P::E], stp[P::E], a[P::E];
+    PixelPhase<RECUR> ph[P::E];
+    double2 a[P::E];
 #pragma unroll
     for (int q = 0; q < P::E; q++) {
         const int iu = P::in_index(t, q);
-        phase_rotation(pp, iu, ycoord, z[q], stp[q]);
+        ph[q].init(pp, pixel_fac(pp, iu, ycoord));
         a[q] = double2{img_re[img_row + iu], img_im[img_row + iu]};
     }
+    T2 twr[NTW > 0 ? NTW : 1];
+    if constexpr (TWREG && NTW > 0) load_twiddles<T2, N, E, 1>(twr, t, tw);
     const long long plane_stride = (long long)ny * N;
     for (int iw = 0; iw < nw; iw++) {
+        const double c2w = 2.0 * LOFAR_REF_PI * (pp.wmin + iw * pp.dw);
         T2 v[P::E];
 #pragma unroll
         for (int q = 0; q < P::E; q++) {
+            const double2 z = ph[q].next(pp, c2w);
             // a * conj(z)
-            const double xr = a[q].x * z[q].x + a[q].y * z[q].y;
-            const double xi = a[q].y * z[q].x - a[q].x * z[q].y;
-            v[q].x = (Real)xr;
-            v[q].y = (Real)xi;
-            z[q] = cmul(z[q], stp[q]);
+            v[q].x = (Real)(a[q].x * z.x + a[q].y * z.y);
+            v[q].y = (Real)(a[q].y * z.x - a[q].x * z.y);
         }
-        stages<T2, N, E, -1, 1>(v, lds, 1, t, tw);
+        if constexpr (TWREG)
+            stages_rt<T2, N, E, -1, 1>(v, lds, t, twr);
+        else
+            stages<T2, N, E, -1, 1>(v, lds, 1, t, tw);
         T2* dst = cube + (long long)iw * plane_stride + img_row;
 #pragma unroll
         for (int e = 0; e < P::E; e++) {
@@ -300,18 +292,6 @@ static int col_cfg()
     return cfg;
 }
 
-template <typename K>
-static int persistent_grid(K kernel, int threads, long long work)
-{
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    long long g = (long long)device_cus() * per_cu;
-    if (g > work) g = work;
-    return (int)(g < 1 ? 1 : g);
-}
-
 template <typename Real, int N>
 static hipError_t launch_rows_n(int sign, const void* in, void* out, const void* tw,
                                 long long lines, double scale, hipStream_t st)
@@ -328,49 +308,83 @@ static hipError_t launch_rows_n(int sign, const void* in, void* out, const void*
     return hipGetLastError();
 }
 
-template <typename Real, int N, int E, int WREQ, bool PERSISTENT>
+template <typename Real, int N, int E, int WREQ, int RREQ>
 static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const void* tw, int nx,
                                   int nw, double scale, hipStream_t st)
 {
     using T2 = C2<Real>;
-    // at most 1024 threads per workgroup
-    constexpr int W = WREQ * Plan<N, E>::TPL <= 1024 ? WREQ : 1024 / Plan<N, E>::TPL;
-    constexpr int threads = W * Plan<N, E>::TPL;
+    constexpr int TPL = Plan<N, E>::TPL;
+    // at most 1024 threads per workgroup; ROUNDS divides W (both powers of two)
+    // and is large enough that one round's columns fit 140 KiB of LDS
+    constexpr int W = WREQ * TPL <= 1024 ? WREQ : 1024 / TPL;
+    constexpr int COL = Plan<N, E>::LDS * 2 * (int)sizeof(Real);
+    constexpr int RMIN = (W * COL + 143359) / 143360;
+    constexpr int RMIN2 = RMIN <= 1 ? 1 : RMIN <= 2 ? 2 : RMIN <= 4 ? 4 : RMIN <= 8 ? 8 : 16;
+    constexpr int R1 = RREQ > RMIN2 ? RREQ : RMIN2;
+    constexpr int ROUNDS = R1 < W ? R1 : W;
     const int strips = (nx + W - 1) / W;
     const long long nstrips = (long long)strips * nw;
-    auto k = PERSISTENT ? (sign > 0 ? k_fft_cols_pf<Real, N, E, W, 1> : k_fft_cols_pf<Real, N, E, W, -1>)
-                        : (sign > 0 ? k_fft_cols<Real, N, E, W, 1> : k_fft_cols<Real, N, E, W, -1>);
-    const long long g = PERSISTENT ? persistent_grid(k, threads, nstrips) : nstrips;
-    hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(threads), 0, st, (const T2*)in, (T2*)out,
+    auto k = sign > 0 ? k_fft_cols<Real, N, E, W, ROUNDS, 1> : k_fft_cols<Real, N, E, W, ROUNDS, -1>;
+    hipLaunchKernelGGL(k, dim3((unsigned)nstrips), dim3(W * TPL), 0, st, (const T2*)in, (T2*)out,
                        (const T2*)tw, nx, strips, nstrips, (Real)scale);
     return hipGetLastError();
 }
 
-// Column-pass configurations by strip width (LDS per strip within 64 or 128 KiB);
-// entry 0 is the default.
+// Column-pass configurations; entry 0 is the default.  W2: strip width whose
+// LDS fits ~128 KiB in one round; W64: 64-byte row segments.
 template <typename Real, int N>
 static hipError_t launch_cols_n(int sign, const void* in, void* out, const void* tw, int nx,
                                 int nw, double scale, hipStream_t st)
 {
     constexpr int B = Plan<N, 16>::LDS * 2 * (int)sizeof(Real);
-    constexpr int W1 = 70000 / B >= 1 ? (70000 / B > 16 ? 16 : 70000 / B) : 1;     // ~64 KiB
-    constexpr int W2 = 140000 / B >= 1 ? (140000 / B > 16 ? 16 : 140000 / B) : 1;  // ~128 KiB
+    constexpr int W2 = 140000 / B >= 1 ? (140000 / B > 16 ? 16 : 140000 / B) : 1;
+    constexpr int W64 = 64 / (2 * (int)sizeof(Real));
     switch (col_cfg()) {
-        case 1: return launch_cols_cfg<Real, N, 16, W1, true>(sign, in, out, tw, nx, nw, scale, st);
-        case 2: return launch_cols_cfg<Real, N, 16, W1, false>(sign, in, out, tw, nx, nw, scale, st);
-        default: return launch_cols_cfg<Real, N, 16, W2, false>(sign, in, out, tw, nx, nw, scale, st);
+        case 1: return launch_cols_cfg<Real, N, 16, W2, 1>(sign, in, out, tw, nx, nw, scale, st);
+        case 2: return launch_cols_cfg<Real, N, 32, W64, 4>(sign, in, out, tw, nx, nw, scale, st);
+        case 3: return launch_cols_cfg<Real, N, 16, W64, 2>(sign, in, out, tw, nx, nw, scale, st);
+        default: return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, st);
     }
 }
 
+static int phase_cfg()
+{
+    static int cfg = -1;
+    if (cfg < 0) {
+        const char* s = getenv("LOFAR_PHASE_CFG");
+        cfg = s ? atoi(s) : 0;
+    }
+    return cfg;
+}
+
+template <typename Real, int N, int E, bool RECUR, bool TWREG>
+static hipError_t launch_rows_phase_cfg(const void* cube, const void* tw, int ny, int nw,
+                                        const PhaseParams& pp, double scale, double* re,
+                                        double* im, int accumulate, hipStream_t st)
+{
+    using T2 = C2<Real>;
+    hipLaunchKernelGGL((k_rows_phase<Real, N, E, RECUR, TWREG>), dim3(ny), dim3(Plan<N, E>::TPL),
+                       0, st, (const T2*)cube, (const T2*)tw, ny, nw, pp, scale, re, im,
+                       accumulate);
+    return hipGetLastError();
+}
+
+// fused row-pass configurations (elements per thread, phase recurrence or direct
+// sincos, twiddles in registers); entry 0 is the default
 template <typename Real, int N>
 static hipError_t launch_rows_phase_n(const void* cube, const void* tw, int ny, int nw,
                                       const PhaseParams& pp, double scale, double* re, double* im,
                                       int accumulate, hipStream_t st)
 {
-    using T2 = C2<Real>;
-    hipLaunchKernelGGL((k_rows_phase<Real, N, kPhaseE>), dim3(ny), dim3(Plan<N, kPhaseE>::TPL), 0,
-                       st, (const T2*)cube, (const T2*)tw, ny, nw, pp, scale, re, im, accumulate);
-    return hipGetLastError();
+#define ARGS_ cube, tw, ny, nw, pp, scale, re, im, accumulate, st
+    switch (phase_cfg()) {
+        case 1: return launch_rows_phase_cfg<Real, N, 8, false, true>(ARGS_);
+        case 2: return launch_rows_phase_cfg<Real, N, 8, true, false>(ARGS_);
+        case 3: return launch_rows_phase_cfg<Real, N, 16, true, true>(ARGS_);
+        case 4: return launch_rows_phase_cfg<Real, N, 4, true, true>(ARGS_);
+        default: return launch_rows_phase_cfg<Real, N, 8, true, true>(ARGS_);
+    }
+#undef ARGS_
 }
 
 template <typename Real, int N>
@@ -379,7 +393,7 @@ static hipError_t launch_rows_phase_adj_n(void* cube, const void* tw, int ny, in
                                           const double* im, hipStream_t st)
 {
     using T2 = C2<Real>;
-    hipLaunchKernelGGL((k_rows_phase_adj<Real, N, kPhaseE>), dim3(ny),
+    hipLaunchKernelGGL((k_rows_phase_adj<Real, N, kPhaseE, true, true>), dim3(ny),
                        dim3(Plan<N, kPhaseE>::TPL), 0, st, (T2*)cube, (const T2*)tw, ny, nw, pp,
                        scale, re, im);
     return hipGetLastError();

@@ -6,27 +6,33 @@ namespace lofar {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
-// Per-line plan: E elements per thread, TPL threads per line, radix-E stages
-// and one smaller remainder stage (last) when log2(N) is not a multiple of log2(E).
+// Per-line plan: E elements per thread, TPL threads per line; stages of radix
+// RX = min(16, E) (E/RX butterflies per thread per stage) and one smaller
+// remainder stage (last) when log2(N) is not a multiple of log2(RX).
 template <int N, int EREQ>
 struct Plan {
     static constexpr int E = N < EREQ ? N : EREQ;
     static constexpr int TPL = N / E;
+    static constexpr int RX = E < 16 ? E : 16;
     static constexpr int LOG = ilog2(N);
-    static constexpr int LOGE = ilog2(E);
-    static constexpr int RL = (N <= E) ? N : ((LOG % LOGE == 0) ? E : (1 << (LOG % LOGE)));
-    // element of the line thread t loads into v[q] before the first stage
-    __device__ static __forceinline__ int in_index(int t, int q) { return t + q * TPL; }
+    static constexpr int LOGR = ilog2(RX);
+    static constexpr int R0 = N < RX ? N : RX;  // first stage
+    static constexpr int RL = (N <= RX) ? N : ((LOG % LOGR == 0) ? RX : (1 << (LOG % LOGR)));
+    // element of the line thread t loads into v[e] before the first stage
+    __device__ static __forceinline__ int in_index(int t, int e)
+    {
+        return t + (e / R0) * TPL + (e % R0) * (N / R0);
+    }
     // element of the line thread t holds in v[e] after the last stage
     __device__ static __forceinline__ int out_index(int t, int e)
     {
         return t + (e / RL) * TPL + (e % RL) * (N / RL);
     }
-    // LDS slot of element i: one pad slot every E elements, so the first
-    // exchange (thread b writes b*E..b*E+E-1) does not put every lane of a
-    // wave on the same banks
-    __device__ static __forceinline__ int pad(int i) { return i + (i >> LOGE); }
-    static constexpr int LDS = N + N / E;  // slots per line
+    // LDS slot of element i: one pad slot every R0 elements, so the first
+    // exchange (butterfly b writes b*R0..b*R0+R0-1) does not put every lane of
+    // a wave on the same banks
+    __device__ static __forceinline__ int pad(int i) { return i + (i >> ilog2(R0)); }
+    static constexpr int LDS = N + N / R0;  // slots per line
 };
 
 // Spread consecutive workgroups of one XCD (blockIdx % 8) over a contiguous range.
@@ -186,15 +192,18 @@ __device__ __forceinline__ void dft(T2* v)
 // Stockham stages from span NS on.  On entry v[m*R+q] holds element b+q*(N/R)
 // of butterfly b = t + m*TPL; on return v holds the outputs in the order of
 // Plan<N,E>::out_index.  `lds` addresses element i of this line at
-// lds[Plan<N,E>::pad(i)*ls] (Plan<N,E>::LDS slots per line).
+// lds[Plan<N,E>::pad(i)*ls] (Plan<N,E>::LDS slots per line).  With ROUNDS > 1
+// the lines of a workgroup take turns on a shared LDS buffer: a thread
+// exchanges in round my_round only (all threads pass every barrier).
 // Twiddles: tw[m] = exp(+2*pi*i*m/N), conjugated for SIGN < 0.
-template <typename T2, int N, int EREQ, int SIGN, int NS>
-__device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* __restrict__ tw)
+template <typename T2, int N, int EREQ, int SIGN, int NS, int ROUNDS = 1>
+__device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* __restrict__ tw,
+                                       int my_round = 0)
 {
     using P = Plan<N, EREQ>;
     constexpr int E = P::E;
     constexpr int TPL = P::TPL;
-    constexpr int R = (N / NS >= E) ? E : N / NS;
+    constexpr int R = (N / NS >= P::RX) ? P::RX : N / NS;
     constexpr int M = E / R;
 #pragma unroll
     for (int m = 0; m < M; m++) {
@@ -209,7 +218,92 @@ __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* 
     }
     if constexpr (NS * R < N) {
         constexpr int NS2 = NS * R;
-        constexpr int R2 = (N / NS2 >= E) ? E : N / NS2;
+        constexpr int R2 = (N / NS2 >= P::RX) ? P::RX : N / NS2;
+        constexpr int M2 = E / R2;
+#pragma unroll
+        for (int rr = 0; rr < ROUNDS; rr++) {
+            __syncthreads();
+            if (rr == my_round) {
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const int b = t + m * TPL;
+                    const int k = b & (NS - 1);
+                    const int base = (b - k) * R + k;
+#pragma unroll
+                    for (int q = 0; q < R; q++) lds[P::pad(base + q * NS) * ls] = v[m * R + q];
+                }
+            }
+            __syncthreads();
+            if (rr == my_round) {
+#pragma unroll
+                for (int m = 0; m < M2; m++) {
+                    const int b = t + m * TPL;
+#pragma unroll
+                    for (int q = 0; q < R2; q++)
+                        v[m * R2 + q] = lds[P::pad(b + q * (N / R2)) * ls];
+                }
+            }
+        }
+        stages<T2, N, EREQ, SIGN, NS2, ROUNDS>(v, lds, ls, t, tw, my_round);
+    }
+}
+
+// Number of twiddle factors one thread applies from span NS to the last stage.
+template <int N, int EREQ>
+constexpr int tw_count(int ns)
+{
+    using P = Plan<N, EREQ>;
+    int total = 0;
+    while (ns < N) {
+        const int r = (N / ns >= P::RX) ? P::RX : N / ns;
+        if (ns > 1) total += (P::E / r) * (r - 1);
+        ns *= r;
+    }
+    return total;
+}
+
+// Loads, once, every twiddle factor thread t applies in stages(), in the order
+// stages_rt() consumes them, so loops over many lines of the same length keep
+// them in registers instead of re-reading the table at every stage.
+template <typename T2, int N, int EREQ, int NS>
+__device__ __forceinline__ void load_twiddles(T2* twr, int t, const T2* __restrict__ tw)
+{
+    using P = Plan<N, EREQ>;
+    constexpr int R = (N / NS >= P::RX) ? P::RX : N / NS;
+    constexpr int M = P::E / R;
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int k = (t + m * P::TPL) & (NS - 1);
+#pragma unroll
+            for (int q = 1; q < R; q++) twr[m * (R - 1) + q - 1] = tw[q * k * (N / (NS * R))];
+        }
+    }
+    if constexpr (NS * R < N)
+        load_twiddles<T2, N, EREQ, NS * R>(twr + (NS > 1 ? M * (R - 1) : 0), t, tw);
+}
+
+// stages() with the twiddle factors taken from registers filled by load_twiddles().
+template <typename T2, int N, int EREQ, int SIGN, int NS>
+__device__ __forceinline__ void stages_rt(T2* v, T2* lds, int t, const T2* twr)
+{
+    using P = Plan<N, EREQ>;
+    constexpr int E = P::E;
+    constexpr int TPL = P::TPL;
+    constexpr int R = (N / NS >= P::RX) ? P::RX : N / NS;
+    constexpr int M = E / R;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int q = 1; q < R; q++)
+                v[m * R + q] = c_mul_tw<SIGN>(v[m * R + q], twr[m * (R - 1) + q - 1]);
+        }
+        dft<R, SIGN>(&v[m * R]);
+    }
+    if constexpr (NS * R < N) {
+        constexpr int NS2 = NS * R;
+        constexpr int R2 = (N / NS2 >= P::RX) ? P::RX : N / NS2;
         constexpr int M2 = E / R2;
         __syncthreads();
 #pragma unroll
@@ -218,17 +312,62 @@ __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* 
             const int k = b & (NS - 1);
             const int base = (b - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) lds[P::pad(base + q * NS) * ls] = v[m * R + q];
+            for (int q = 0; q < R; q++) lds[P::pad(base + q * NS)] = v[m * R + q];
         }
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < M2; m++) {
             const int b = t + m * TPL;
 #pragma unroll
-            for (int q = 0; q < R2; q++) v[m * R2 + q] = lds[P::pad(b + q * (N / R2)) * ls];
+            for (int q = 0; q < R2; q++) v[m * R2 + q] = lds[P::pad(b + q * (N / R2))];
         }
-        stages<T2, N, EREQ, SIGN, NS2>(v, lds, ls, t, tw);
+        stages_rt<T2, N, EREQ, SIGN, NS2>(v, lds, t, twr + (NS > 1 ? M * (R - 1) : 0));
     }
+}
+
+// sin and cos of x in double precision, from the mathematics: x = n*pi/2 + t
+// with a three-part Cody-Waite pi/2 (the first two parts have 33 significant
+// bits, so n*part is exact for |n| < 2^20), then Taylor series to t^17 / t^16 on
+// |t| <= pi/4 (truncation below 1e-18).  |x| beyond 1e5 falls back to sincos().
+constexpr double kPio2Hi = 1.5707963267341256;
+constexpr double kPio2Mid = 6.077100506303966e-11;
+constexpr double kPio2Lo = 2.0222662487959506e-21;
+constexpr double kTwoOverPi = 0.6366197723675814;
+__device__ __forceinline__ void sincos_w(double x, double* sn, double* cs)
+{
+    if (!(fabs(x) < 1.0e5)) {
+        sincos(x, sn, cs);
+        return;
+    }
+    const double n = rint(x * kTwoOverPi);
+    double t = fma(-n, kPio2Hi, x);
+    t = fma(-n, kPio2Mid, t);
+    t = fma(-n, kPio2Lo, t);
+    const double t2 = t * t;
+    // 1/k! for k = 3..17 (odd) and 2..16 (even), Horner in t^2
+    double ps = 2.8114572543455206e-15;
+    ps = fma(ps, t2, -7.647163731819816e-13);
+    ps = fma(ps, t2, 1.6059043836821613e-10);
+    ps = fma(ps, t2, -2.505210838544172e-08);
+    ps = fma(ps, t2, 2.7557319223985893e-06);
+    ps = fma(ps, t2, -0.0001984126984126984);
+    ps = fma(ps, t2, 0.008333333333333333);
+    ps = fma(ps, t2, -0.16666666666666666);
+    const double st = fma(ps * t2, t, t);
+    double pc = 4.779477332387385e-14;
+    pc = fma(pc, t2, -1.1470745597729725e-11);
+    pc = fma(pc, t2, 2.08767569878681e-09);
+    pc = fma(pc, t2, -2.755731922398589e-07);
+    pc = fma(pc, t2, 2.48015873015873e-05);
+    pc = fma(pc, t2, -0.001388888888888889);
+    pc = fma(pc, t2, 0.041666666666666664);
+    pc = fma(pc, t2, -0.5);
+    const double ct = fma(pc, t2, 1.0);
+    const int q = ((int)n) & 3;
+    const double s_ = (q & 1) ? ct : st;
+    const double c_ = (q & 1) ? st : ct;
+    *sn = (q & 2) ? -s_ : s_;
+    *cs = ((q + 1) & 2) ? -c_ : c_;
 }
 
 // phase constants of phase_correction.cu:92-94 (phase.hip)

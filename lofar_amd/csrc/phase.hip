@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_phase_sum(const typename cplx<Real>::ty
         if (pp.phase_on) {
             const double wterm = pp.wmin + iw * pp.dw;
             double s, r;
-            sincos(2.0 * LOFAR_REF_PI * wterm * (fac - 1.0), &s, &r);
+            sincos_w(2.0 * LOFAR_REF_PI * wterm * (fac - 1.0), &s, &r);
             sr += (p * r - q * s) * pp.dwnorm * fac;
             si += (p * s + q * r) * pp.dwnorm * fac;
         } else {
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_phase_adjoint(const double* img_re, con
         if (pp.phase_on) {
             const double wterm = pp.wmin + iw * pp.dw;
             double s, r;
-            sincos(2.0 * LOFAR_REF_PI * wterm * (fac - 1.0), &s, &r);
+            sincos_w(2.0 * LOFAR_REF_PI * wterm * (fac - 1.0), &s, &r);
             const double f = pp.dwnorm * fac;
             xr = (a * r + b * s) * f;
             xi = (b * r - a * s) * f;
