@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -204,34 +205,82 @@ Geometry make_geometry(int nw, int support, int nx, int ny, double dx, double dw
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Counting sort of the samples into (w, tile) buckets; validates every sample
-// and returns the number of entries.  The only host synchronisation of a call.
+// Bucket-sort tuning: super-tile side (tiles) and samples per level-1
+// workgroup.  LOFAR_BUCKET_CFG="sst,chunk" overrides, for measurement.
+void bucket_cfg(int& sst, long long& chunk)
+{
+    static int s_sst = 0;
+    static long long s_chunk = 0;
+    if (!s_sst) {
+        int a = 4;
+        long long b = 16384;
+        if (const char* e = getenv("LOFAR_BUCKET_CFG")) sscanf(e, "%d,%lld", &a, &b);
+        if (a < 1 || a > 16 || (a & (a - 1))) a = 16;
+        if (b < 256) b = 256;
+        s_chunk = b;
+        s_sst = a;
+    }
+    sst = s_sst;
+    chunk = s_chunk;
+}
+
+// Two-level counting sort of the samples into (w, tile) buckets: sizes the
+// workspace, counts per super-tile, scans, and reads back the entry total and
+// the validation flag (the only host synchronisation of a call).  The level-1
+// histogram of nw*ceil(ntv/sst)*ceil(ntu/sst) super-tiles lives in LDS and is
+// limited to kMaxSuperTiles (64 KiB).
+constexpr long long kMaxSuperTiles = 16384;
+
 int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double* vv,
                   const double* ww, const Geometry& g, hipStream_t st, Buckets& b,
                   long long& total)
 {
-    b.nb = (long long)g.nw * g.ntv * g.ntu;
-    const long long nblk = (b.nb + 4095) / 4096;
+    int sst;
+    long long chunk;
+    bucket_cfg(sst, chunk);
+    const int span = g.ntu > g.ntv ? g.ntu : g.ntv;
+    while (sst > 1 && sst / 2 >= span) sst /= 2;  // no super-tile wider than the grid
+    auto super_tiles = [&](int s) {
+        return (long long)g.nw * ((g.ntv + s - 1) / s) * ((g.ntu + s - 1) / s);
+    };
+    while (sst < 16 && super_tiles(sst) > kMaxSuperTiles) sst *= 2;  // large grids
+    TileMap& m = b.map;
+    m.sst = sst;
+    m.lsst = 0;
+    while ((1 << m.lsst) < sst) m.lsst++;
+    m.nstu = (g.ntu + sst - 1) / sst;
+    m.nstv = (g.ntv + sst - 1) / sst;
+    m.nk1 = super_tiles(sst);
+    if (m.nk1 > kMaxSuperTiles) {
+        fprintf(stderr,
+                "liblofar_hip: %d w-planes of %dx%d cells need %lld super-tiles of %dx%d uv "
+                "tiles; the bucket sort handles at most %lld\n",
+                g.nw, g.nx, g.ny, m.nk1, sst, sst, kMaxSuperTiles);
+        return fail(LOFAR_ESIZE);
+    }
+    b.nwg = n > 0 ? (n + chunk - 1) / chunk : 1;
+    if (b.nwg > 4096) b.nwg = 4096;
+    b.chunk = n > 0 ? (n + b.nwg - 1) / b.nwg : 1;
+    const long long nblk = (m.nk1 + 4095) / 4096;
     const size_t off_status = 0;
-    const size_t off_counts = align256(off_status + 2 * sizeof(long long));
-    const size_t off_offsets = align256(off_counts + (size_t)b.nb * sizeof(unsigned));
-    const size_t off_partials = align256(off_offsets + (size_t)(b.nb + 1) * sizeof(long long));
-    const size_t off_slots = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
-    const size_t bytes = align256(off_slots + (size_t)(n > 0 ? n : 1) * 4 * sizeof(unsigned));
+    const size_t off_count1 = align256(off_status + 2 * sizeof(long long));
+    const size_t off_off1 = align256(off_count1 + (size_t)m.nk1 * sizeof(unsigned));
+    const size_t off_base1 = align256(off_off1 + (size_t)(m.nk1 + 1) * sizeof(long long));
+    const size_t off_offsets = align256(off_base1 + (size_t)(b.nwg * m.nk1) * sizeof(unsigned));
+    const size_t off_partials =
+        align256(off_offsets + (size_t)(m.nbuckets() + 1) * sizeof(long long));
+    const size_t bytes = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
     int r = ensure(c, c.meta, bytes);
     if (r) return r;
     char* base = (char*)c.meta.ptr;
     b.total = (long long*)(base + off_status);
     b.err = (int*)(base + off_status + sizeof(long long));
-    b.counts = (unsigned*)(base + off_counts);
+    b.count1 = (unsigned*)(base + off_count1);
+    b.off1 = (long long*)(base + off_off1);
+    b.base1 = (unsigned*)(base + off_base1);
     b.offsets = (long long*)(base + off_offsets);
     b.partials = (long long*)(base + off_partials);
-    b.slots = (unsigned*)(base + off_slots);
-    HIPCHK(stage(c, kStCount, st, [&] {
-        hipError_t e = bucket_zero(b, st);
-        if (e != hipSuccess) return e;
-        return bucket_count(mode, n, uu, vv, ww, g, b, st);
-    }));
+    HIPCHK(stage(c, kStCount, st, [&] { return bucket_count(mode, n, uu, vv, ww, g, b, st); }));
     HIPCHK(stage(c, kStScan, st, [&] { return bucket_scan(b, st); }));
     HIPCHK(hipMemcpyAsync(c.status_host, b.total, 2 * sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -254,15 +303,20 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     long long total = 0;
     r = build_buckets(c, kBucketFootprint, n, uu, vv, ww, g, st, b, total);
     if (r) return r;
+    // level-1 records, their windows, the sorted entries and their windows
     const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
-    const size_t off_wins = align256((size_t)total * esz);
-    r = ensure(c, c.ents, off_wins + (size_t)total * sizeof(uint32_t) + 256);
+    const size_t ebytes = align256((size_t)total * esz);
+    const size_t wbytes = align256((size_t)total * sizeof(uint32_t));
+    r = ensure(c, c.ents, 2 * (ebytes + wbytes) + 256);
     if (r) return r;
-    void* ents = c.ents.ptr;
-    uint32_t* wins = (uint32_t*)((char*)c.ents.ptr + off_wins);
+    char* eb = (char*)c.ents.ptr;
+    void* rec = eb;
+    uint32_t* recwin = (uint32_t*)(eb + ebytes);
+    void* ents = eb + ebytes + wbytes;
+    uint32_t* wins = (uint32_t*)(eb + 2 * ebytes + wbytes);
     HIPCHK(stage(c, kStScatter, st, [&] {
-        return bucket_scatter_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, ents, wins,
-                                   st);
+        return bucket_sort_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, rec, recwin,
+                                ents, wins, st);
     }));
     HIPCHK(stage(c, kStGrid, st,
                  [&] { return grid_tiles(precision, g, b, ents, wins, cube, accumulate, st); }));
@@ -524,11 +578,13 @@ int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long
     long long total = 0;
     r = build_buckets(c, kBucketHome, num_points, uu, vv, ww, g, cg.st, b, total);
     if (r) return r;
-    r = ensure(c, c.ents, (size_t)total * sizeof(DEntry) + 256);
+    r = ensure(c, c.ents, 2 * (size_t)total * sizeof(DEntry) + 256);
     if (r) return r;
-    DEntry* ents = (DEntry*)c.ents.ptr;
-    HIPCHK(stage(c, kStScatter, cg.st,
-                 [&] { return bucket_scatter_home(num_points, uu, vv, ww, g, b, ents, cg.st); }));
+    DEntry* rec = (DEntry*)c.ents.ptr;
+    DEntry* ents = rec + total;
+    HIPCHK(stage(c, kStScatter, cg.st, [&] {
+        return bucket_sort_home(num_points, uu, vv, ww, g, b, rec, ents, cg.st);
+    }));
     HIPCHK(stage(c, kStDegrid, cg.st, [&] {
         return degrid_tiles(precision, g, b, ents, cube, weight, freq_per_chan, polarizations,
                             model_real, model_imag, cg.st);

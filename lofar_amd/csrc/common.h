@@ -27,10 +27,12 @@ struct Geometry {
     double dx, dw;      // cell size in normalised uv units, plane width in w units
 };
 
-// Packs a per-tile clipped window [a0,a1] x [b0,b1] (tile-local, each < 256).
-__host__ __device__ inline uint32_t pack_win(int a0, int a1, int b0, int b1)
+// Packs a per-tile clipped window [a0,a1] x [b0,b1] (tile-local, each < 64)
+// and, in the top byte, the tile's index inside its super-tile (bucket sort).
+__host__ __device__ inline uint32_t pack_win(int a0, int a1, int b0, int b1, int fine = 0)
 {
-    return (uint32_t)a0 | ((uint32_t)a1 << 8) | ((uint32_t)b0 << 16) | ((uint32_t)b1 << 24);
+    return (uint32_t)a0 | ((uint32_t)a1 << 6) | ((uint32_t)b0 << 12) | ((uint32_t)b1 << 18) |
+           ((uint32_t)fine << 24);
 }
 
 // Gridding entry, FP64 mode: absolute grid position (uu/dx, vv/dx) and the

@@ -31,29 +31,64 @@ __device__ __forceinline__ bool sample_window(double u, double v, double w, cons
     return true;
 }
 
-// Workspace layout for one bucketing of n samples into nb tiles.
+// Bucket order of the (w-plane, tile) buckets: super-tiles of sst x sst tiles,
+// (w, super-row, super-column) major, then the fine tile inside the super-tile.
+struct TileMap {
+    int sst, lsst;   // super-tile side in tiles (power of two) and its log2
+    int nstu, nstv;  // super-tiles along u and v
+    long long nk1;   // super-tile buckets, nw*nstv*nstu
+    __host__ __device__ long long key1(int w, int tv, int tu) const
+    {
+        return ((long long)w * nstv + (tv >> lsst)) * nstu + (tu >> lsst);
+    }
+    __host__ __device__ int fine(int tv, int tu) const
+    {
+        return ((tv & (sst - 1)) << lsst) | (tu & (sst - 1));
+    }
+    __host__ __device__ long long nbuckets() const { return nk1 << (2 * lsst); }
+    // tile of bucket bkt; false for the slots of partial super-tiles beyond the grid
+    __device__ bool decode(long long bkt, const Geometry& g, long long& w, int& tv, int& tu) const
+    {
+        const long long k1 = bkt >> (2 * lsst);
+        const int f = (int)(bkt & ((1 << (2 * lsst)) - 1));
+        const int stu = (int)(k1 % nstu);
+        const long long r = k1 / nstu;
+        const int stv = (int)(r % nstv);
+        w = r / nstv;
+        tv = (stv << lsst) + (f >> lsst);
+        tu = (stu << lsst) + (f & (sst - 1));
+        return tv < g.ntv && tu < g.ntu;
+    }
+};
+
+// Workspace of one bucketing of n samples (two-level counting sort).
 struct Buckets {
-    long long nb;          // number of (w, tile_v, tile_u) buckets
-    unsigned* counts;      // [nb]
-    unsigned* slots;       // [4n]: position of each (sample, tile) pair within its bucket
-    long long* offsets;    // [nb+1]
+    TileMap map;
+    long long nwg, chunk;  // level-1 workgroups and samples per workgroup
+    unsigned* count1;      // [nk1] entries per super-tile
+    long long* off1;       // [nk1+1]
+    unsigned* base1;       // [nwg*nk1] each workgroup's first slot in each super-tile
+    long long* offsets;    // [nbuckets+1] entries of every (w, tile) bucket, bucket order
     long long* partials;   // scan partials
     long long* total;      // host-visible total entry count (status[0])
     int* err;              // host-visible validation flag (status[1])
 };
 
+enum { kBucketFootprint = 0, kBucketHome = 1 };
+
+// Level 1 (bucket.hip): count the (sample, tile) pairs per super-tile.
 hipError_t bucket_count(int mode, long long n, const double* uu, const double* vv, const double* ww,
                         const Geometry& g, const Buckets& b, hipStream_t st);
 hipError_t bucket_scan(const Buckets& b, hipStream_t st);
-hipError_t bucket_zero(const Buckets& b, hipStream_t st);
-hipError_t bucket_scatter_grid(int precision, long long n, long long F, long long P,
-                               const double* uu, const double* vv, const double* ww,
-                               const float* vr, const float* vi, const float* wt,
-                               const Geometry& g, const Buckets& b, void* entries,
-                               uint32_t* wins, hipStream_t st);
-hipError_t bucket_scatter_home(long long n, const double* uu, const double* vv, const double* ww,
-                               const Geometry& g, const Buckets& b, DEntry* entries,
-                               hipStream_t st);
+// Level 1 scatter into super-tile order, then level 2 into (w, tile) order.
+hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P,
+                            const double* uu, const double* vv, const double* ww, const float* vr,
+                            const float* vi, const float* wt, const Geometry& g, const Buckets& b,
+                            void* rec, uint32_t* recwin, void* entries, uint32_t* wins,
+                            hipStream_t st);
+hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
+                            const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
+                            hipStream_t st);
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
                       const uint32_t* wins, void* cube, int accumulate, hipStream_t st);
 hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, const DEntry* entries,
@@ -66,6 +101,6 @@ hipError_t phase_adjoint(int precision, const double* re, const double* im, void
                          int yaxis, int nw, int xtot, int ytot, double resolution, double wmin,
                          double wmax, hipStream_t st);
 
-enum { kBucketFootprint = 0, kBucketHome = 1 };
+
 
 }  // namespace lofar
