@@ -94,10 +94,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (3 launches), the "
+                         "known-size reference for scaling HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the w-plane sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: liblofar_hip has no multi-GPU path "
+              "(the path would shard by w-plane with one image reduction)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -106,6 +109,15 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.calibrate_copy:
+        src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 8,
+                          "copy_bytes_written_per_launch": dst.numel() * 8}), flush=True)
+        return
     nx = ny = 4096
     nw, support = 32, 7
     n2 = 16 * 1024 * 1024

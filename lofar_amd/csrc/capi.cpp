@@ -77,7 +77,7 @@ int get_ctx(DeviceCtx** out)
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return LOFAR_ENODEV;
         if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-            fprintf(stderr, "liblofar_hip: device %d is %s, this build targets gfx950\n", dev,
+            fprintf(stderr, "liblofar_hip: device %d is %s; the library is compiled for gfx950 only\n", dev,
                     prop.gcnArchName);
             return LOFAR_ENODEV;
         }

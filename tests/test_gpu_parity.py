@@ -2,7 +2,7 @@
 golden fixtures, at sizes the oracle finishes in seconds.
 
 Tolerances (relative L2 over the whole array unless stated):
-  FP64 mode: cube 1e-12, plane transform 1e-13, image 1e-10 (BASELINE.json target);
+  FP64 mode: cube 1e-12, plane transform 1e-13, image 1e-10 (the FP64 parity bound);
   FP32C mode: cube 1e-6, plane transform 2e-6, image 1e-6.
 """
 import glob
@@ -97,6 +97,63 @@ def test_grid_matches_oracle(cuda, seed, n, nx, ny, nw, support, F, P, edge, pre
     cube = imager.grid(*dev_inputs(d, cuda), nw, support, nx, ny, precision=precision,
                        freq_per_chan=F, polarizations=P)
     assert rel(H(cube), want) < tol
+
+
+def test_fp32c_grid_is_bitwise_order_independent(cuda):
+    """FP32C tiles accumulate exact fixed-point sums: any sample order gives the same bits."""
+    from lofar_amd import imager
+    d = random_case(11, 60000, 192, 160, 4, F=2, P=1, edge=True)
+    perm = np.random.default_rng(12).permutation(60000)
+    dp = dict(d)
+    for k in ("uu", "vv", "ww"):
+        dp[k] = d[k][perm]
+    for k in ("vis_real", "vis_img"):
+        dp[k] = d[k].reshape(60000, 2)[perm].reshape(-1)
+    dp["weight"] = d["weight"][perm]
+    a = H(imager.grid(*dev_inputs(d, cuda), 4, 7, 192, 160, precision="fp32c",
+                      freq_per_chan=2))
+    b = H(imager.grid(*dev_inputs(dp, cuda), 4, 7, 192, 160, precision="fp32c",
+                      freq_per_chan=2))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    st, want = O.wstack(4, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / 192, 1 / 4, 7, 192, 160, freq_per_chan=2)
+    assert st == O.OK and rel(a, want) < 1e-6
+
+
+_BUCKET_SCRIPT = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle as O
+from lofar_amd import imager
+from tests.test_gpu_parity import random_case, dev_inputs, rel, H
+dev = torch.device("cuda:0")
+for seed, n, nx, ny, nw in ((21, 40000, 300, 200, 3), (22, 3000, 1100, 64, 2)):
+    d = random_case(seed, n, nx, ny, nw)
+    st, want = O.wstack(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / nx, 1 / nw, 7, nx, ny)
+    assert st == O.OK
+    for prec, tol in (("fp64", 1e-12), ("fp32c", 1e-6)):
+        e = rel(H(imager.grid(*dev_inputs(d, dev), nw, 7, nx, ny, precision=prec)), want)
+        assert e < tol, (seed, prec, e)
+    uu, vv, ww, _, _, wt = dev_inputs(d, dev)
+    mr, mi = imager.degrid_cube(torch.from_numpy(want.copy()).to(dev), uu, vv, ww, wt, 7)
+    st, model = O.degrid(want, d["uu"], d["vv"], d["ww"], d["weight"], 1 / nx, 1 / nw, 7)
+    assert st == O.OK and rel(H(mr) + 1j * H(mi), model) < 1e-12
+print("BUCKETS_OK")
+"""
+
+
+@pytest.mark.parametrize("cfg", ["1,256", "2,1000", "16,300", "8,1000000"])
+def test_bucket_sort_configurations(cuda, cfg):
+    """Other super-tile sides and level-1 chunk sizes (LOFAR_BUCKET_CFG, read once per
+    process) give the same cube; run in a child process with the setting applied."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, LOFAR_BUCKET_CFG=cfg)
+    r = subprocess.run([sys.executable, "-c", _BUCKET_SCRIPT, root], env=env, cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "BUCKETS_OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
 
 
 def test_grid_accumulate_and_empty(cuda):
