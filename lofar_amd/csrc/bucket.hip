@@ -1,26 +1,43 @@
 // Bucketing of the samples into (w-plane, uv tile) buckets for gfx950: a
 // two-level counting sort whose histograms live in LDS.
 //
-// Level 1 groups the (sample, tile) pairs by super-tile (sst x sst tiles of one
-// w-plane).  Every workgroup takes a contiguous chunk of samples, counts its
-// pairs per super-tile in an LDS histogram and reserves its range in each
-// super-tile with one global atomic per non-empty bin (instead of one per pair),
-// then, after the scan, places its pairs with LDS atomics, so each workgroup
-// writes runs of consecutive entries.  Level 2 gives each super-tile one
-// workgroup that counts its entries per fine tile in LDS and writes them in
-// (w, tile) bucket order.  The entries carry what the gridder needs: the
-// window clipped to the tile (w-stacking_omp.c:93-96) and the sample's
-// position and channel/polarisation-summed visibility (:85-90,116-133).
+// Level 1 groups the samples by super-tile (sst x sst tiles of one w-plane):
+// one 32-byte record (position and channel/polarisation-summed visibility,
+// w-stacking_omp.c:85-90,116-133) per super-tile the sample's window touches,
+// which is one record for almost every sample.  Every workgroup takes a
+// contiguous chunk of samples, counts its records (and the tile entries they
+// will expand into) per super-tile in LDS histograms, reserves its range in
+// each super-tile with one global atomic per non-empty bin, and after the scan
+// places its records with LDS cursors.
+//
+// Level 2 splits every super-tile's records into blocks of at most l2_block
+// records, so that the work spreads over many workgroups even when the w
+// distribution piles samples into a few planes: a per-block LDS count of the
+// entries per tile, a per-super-tile scan over its blocks (which also yields
+// the bucket offsets), then a per-block scatter with LDS cursors that expands
+// every record into one entry per tile of its window (w-stacking_omp.c:93-96).
+// Entries are self-contained (no side arrays), so every scattered write is one
+// aligned 16- or 32-byte store.
 #include "grid_kernels.h"
 
 namespace lofar {
 
 constexpr int kScanBlock = 4096;  // elements per block of the local scan (256 threads x 16)
+constexpr int kMaxFine = 256;     // sst <= 16
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, long long n, unsigned v)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Tiles [tu_a, tu_b] x [tv_a, tv_b] of a non-empty window.
+struct TileSpan {
+    int tu_a, tu_b, tv_a, tv_b;
+};
+__device__ __forceinline__ TileSpan tiles_of(const Window& s)
+{
+    return TileSpan{s.j0 / kTile, s.j1 / kTile, s.k0 / kTile, s.k1 / kTile};
 }
 
 // ---------------------------------------------------------------------------
@@ -29,11 +46,12 @@ __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, long long n, unsi
 __global__ __launch_bounds__(256) void k_l1_count(int mode, long long n, const double* uu,
                                                   const double* vv, const double* ww, Geometry g,
                                                   TileMap tm, long long chunk, unsigned* count1,
-                                                  unsigned* base1, int* err)
+                                                  unsigned* count1e, unsigned* base1, int* err)
 {
-    extern __shared__ unsigned hist[];
+    extern __shared__ unsigned hist[];  // [nk1] records, then [nk1] entries
+    unsigned* hent = hist + tm.nk1;
     const int tid = threadIdx.x;
-    for (long long k = tid; k < tm.nk1; k += 256) hist[k] = 0u;
+    for (long long k = tid; k < 2 * tm.nk1; k += 256) hist[k] = 0u;
     __syncthreads();
     const long long beg = (long long)blockIdx.x * chunk;
     const long long end = min(n, beg + chunk);
@@ -45,26 +63,38 @@ __global__ __launch_bounds__(256) void k_l1_count(int mode, long long n, const d
             continue;
         }
         if (mode == kBucketHome) {
-            const int cu = min(max(s.gu, 0), g.nx - 1), cv = min(max(s.gv, 0), g.ny - 1);
-            atomicAdd(&hist[tm.key1(s.gw, cv / kTile, cu / kTile)], 1u);
+            int cu, cv;
+            home_cell(s, g, cu, cv);
+            const long long k = tm.key1(s.gw, cv / kTile, cu / kTile);
+            atomicAdd(&hist[k], 1u);
+            atomicAdd(&hent[k], 1u);
             continue;
         }
         if (s.j0 > s.j1 || s.k0 > s.k1) continue;
-        for (int tv = s.k0 / kTile; tv <= s.k1 / kTile; tv++)
-            for (int tu = s.j0 / kTile; tu <= s.j1 / kTile; tu++)
-                atomicAdd(&hist[tm.key1(s.gw, tv, tu)], 1u);
+        const TileSpan t = tiles_of(s);
+        for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++) {
+            const int nv = min(t.tv_b, (sv << tm.lsst) + tm.sst - 1) - max(t.tv_a, sv << tm.lsst) + 1;
+            for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
+                const int nu =
+                    min(t.tu_b, (su << tm.lsst) + tm.sst - 1) - max(t.tu_a, su << tm.lsst) + 1;
+                const long long k = tm.key_st(s.gw, sv, su);
+                atomicAdd(&hist[k], 1u);
+                atomicAdd(&hent[k], (unsigned)(nu * nv));
+            }
+        }
     }
     if (bad) atomicOr(err, 1);
     __syncthreads();
     unsigned* my_base = base1 + (long long)blockIdx.x * tm.nk1;
     for (long long k = tid; k < tm.nk1; k += 256) {
-        const unsigned c = hist[k];
+        const unsigned c = hist[k], ce = hent[k];
         my_base[k] = c ? atomicAdd(&count1[k], c) : 0u;
+        if (ce) atomicAdd(&count1e[k], ce);
     }
 }
 
 // ---------------------------------------------------------------------------
-// exclusive scan (counts -> offsets; total to offsets[n] and to the status word)
+// exclusive scan (counts -> offsets; the total to two places)
 
 __global__ __launch_bounds__(256) void k_scan_local(const unsigned* in, long long* out,
                                                     long long* partials, long long n)
@@ -133,7 +163,7 @@ __global__ __launch_bounds__(256) void k_scan_add(long long* out, const long lon
 }
 
 // ---------------------------------------------------------------------------
-// level 1: scatter into super-tile order
+// level 1: scatter records into super-tile order
 
 // Channel/polarisation sum of one sample with the reference's weighting:
 // float product weight*vis, promoted to double, NaN real parts skipped
@@ -160,15 +190,13 @@ __device__ __forceinline__ void sample_vis(long long i, long long F, long long P
     }
 }
 
-template <bool FP64>
 __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long F, long long P,
                                                          const double* uu, const double* vv,
                                                          const double* ww, const float* vr,
                                                          const float* vi, const float* wt,
                                                          Geometry g, TileMap tm, long long chunk,
                                                          const long long* off1,
-                                                         const unsigned* base1, void* rec,
-                                                         uint32_t* recwin)
+                                                         const unsigned* base1, Entry64* rec)
 {
     extern __shared__ unsigned cur[];
     const int tid = threadIdx.x;
@@ -181,34 +209,16 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
         SampleWin s;
         if (!sample_window(uu[i], vv[i], ww[i], g, s)) continue;
         if (s.j0 > s.j1 || s.k0 > s.k1) continue;
-        double sr, si;
-        sample_vis(i, F, P, vr, vi, wt, sr, si);
-        for (int tv = s.k0 / kTile; tv <= s.k1 / kTile; tv++) {
-            const int v0 = tv * kTile;
-            const int b0 = max(s.k0, v0) - v0, b1 = min(s.k1, v0 + kTile - 1) - v0;
-            for (int tu = s.j0 / kTile; tu <= s.j1 / kTile; tu++) {
-                const int u0 = tu * kTile;
-                const int a0 = max(s.j0, u0) - u0, a1 = min(s.j1, u0 + kTile - 1) - u0;
-                const long long k1 = tm.key1(s.gw, tv, tu);
-                const long long pos = off1[k1] + atomicAdd(&cur[k1], 1u);
-                recwin[pos] = pack_win(a0, a1, b0, b1, tm.fine(tv, tu));
-                if constexpr (FP64) {
-                    Entry64 e;
-                    e.pu = s.pu;
-                    e.pv = s.pv;
-                    e.vr = sr;
-                    e.vi = si;
-                    ((Entry64*)rec)[pos] = e;
-                } else {
-                    Entry32 e;
-                    e.du0 = (float)((double)(u0 + a0) + 0.5 - s.pu);
-                    e.dv0 = (float)((double)(v0 + b0) + 0.5 - s.pv);
-                    e.vr = (float)sr;
-                    e.vi = (float)si;
-                    ((Entry32*)rec)[pos] = e;
-                }
+        Entry64 e;
+        e.pu = s.pu;
+        e.pv = s.pv;
+        sample_vis(i, F, P, vr, vi, wt, e.vr, e.vi);
+        const TileSpan t = tiles_of(s);
+        for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++)
+            for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
+                const long long k = tm.key_st(s.gw, sv, su);
+                rec[off1[k] + atomicAdd(&cur[k], 1u)] = e;
             }
-        }
     }
 }
 
@@ -228,70 +238,178 @@ __global__ __launch_bounds__(256) void k_l1_scatter_home(long long n, const doub
     for (long long i = beg + tid; i < end; i += 256) {
         SampleWin s;
         if (!sample_window(uu[i], vv[i], ww[i], g, s)) continue;
-        const int cu = min(max(s.gu, 0), g.nx - 1), cv = min(max(s.gv, 0), g.ny - 1);
-        const int tv = cv / kTile, tu = cu / kTile;
-        const long long k1 = tm.key1(s.gw, tv, tu);
-        const long long pos = off1[k1] + atomicAdd(&cur[k1], 1u);
+        int cu, cv;
+        home_cell(s, g, cu, cv);
+        const long long k = tm.key1(s.gw, cv / kTile, cu / kTile);
         DEntry e;
         e.pu = s.pu;
         e.pv = s.pv;
         e.idx = i;
-        e.pad = tm.fine(tv, tu);
-        rec[pos] = e;
+        e.pad = 0;
+        rec[off1[k] + atomicAdd(&cur[k], 1u)] = e;
     }
 }
 
 // ---------------------------------------------------------------------------
-// level 2: one workgroup per super-tile, fine-tile order
+// level 2
 
-constexpr int kMaxFine = 256;  // sst <= 16: the fine index fills the top byte of a packed window
+enum { kOutFp64 = 0, kOutFp32 = 1, kOutHome = 2 };
 
-// fine-tile index of a level-1 record: in DEntry.pad (home) or in the top byte
-// of the packed window (footprint)
-__device__ __forceinline__ int fine_of(const DEntry* rec, const uint32_t*, long long r)
+// Tiles of the record's window inside super-tile (sv, su), as inclusive ranges
+// of fine coordinates; for the home mode the single home tile.
+template <int OUT, typename Rec>
+__device__ __forceinline__ bool record_tiles(const Rec& r, const Geometry& g, const TileMap& tm,
+                                             int sv, int su, int& fu_a, int& fu_b, int& fv_a,
+                                             int& fv_b, Window& s)
 {
-    return (int)rec[r].pad;
-}
-template <typename Rec>
-__device__ __forceinline__ int fine_of(const Rec*, const uint32_t* recwin, long long r)
-{
-    return (int)(recwin[r] >> 24);
+    s = window_of(r.pu, r.pv, g);
+    if constexpr (OUT == kOutHome) {
+        int cu, cv;
+        home_cell(s, g, cu, cv);
+        fu_a = fu_b = (cu / kTile) - (su << tm.lsst);
+        fv_a = fv_b = (cv / kTile) - (sv << tm.lsst);
+        return true;
+    } else {
+        const TileSpan t = tiles_of(s);
+        fu_a = max(t.tu_a - (su << tm.lsst), 0);
+        fu_b = min(t.tu_b - (su << tm.lsst), tm.sst - 1);
+        fv_a = max(t.tv_a - (sv << tm.lsst), 0);
+        fv_b = min(t.tv_b - (sv << tm.lsst), tm.sst - 1);
+        return fu_a <= fu_b && fv_a <= fv_b;
+    }
 }
 
-template <typename Rec, bool HOME>
-__global__ __launch_bounds__(256) void k_l2_sort(TileMap tm, const long long* off1,
-                                                 const Rec* rec, const uint32_t* recwin, Rec* ent,
-                                                 uint32_t* entwin, long long* offsets)
+// blocks per segment
+__global__ __launch_bounds__(256) void k_l2_nblocks(long long nk1, const long long* off1,
+                                                    long long bs, unsigned* nblk)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nk1) nblk[k] = (unsigned)((off1[k + 1] - off1[k] + bs - 1) / bs);
+}
+
+// block descriptors: segment, first record, end record
+__global__ __launch_bounds__(256) void k_l2_blocks(long long nk1, const long long* off1,
+                                                   long long bs, const long long* boff,
+                                                   L2Block* blk)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk1) return;
+    const long long beg = off1[k], end = off1[k + 1];
+    long long j = boff[k];
+    for (long long r = beg; r < end; r += bs, j++) {
+        L2Block d;
+        d.k1 = k;
+        d.beg = r;
+        d.end = min(end, r + bs);
+        d.pad = 0;
+        blk[j] = d;
+    }
+}
+
+template <int OUT, typename Rec>
+__global__ __launch_bounds__(256) void k_l2_count(Geometry g, TileMap tm, const long long* nblocks,
+                                                  const L2Block* blk, const Rec* rec,
+                                                  unsigned* cnt2)
 {
     __shared__ unsigned cnt[kMaxFine];
+    const long long b = blockIdx.x;
+    if (b >= *nblocks) return;  // the grid is sized from an upper bound
+    const int tid = threadIdx.x;
+    const int nf = tm.sst * tm.sst;
+    for (int f = tid; f < nf; f += 256) cnt[f] = 0u;
+    __syncthreads();
+    const L2Block d = blk[b];
+    int sv, su;
+    tm.super_of(d.k1, sv, su);
+    for (long long r = d.beg + tid; r < d.end; r += 256) {
+        int fu_a, fu_b, fv_a, fv_b;
+        Window s;
+        if (!record_tiles<OUT>(rec[r], g, tm, sv, su, fu_a, fu_b, fv_a, fv_b, s)) continue;
+        for (int fv = fv_a; fv <= fv_b; fv++)
+            for (int fu = fu_a; fu <= fu_b; fu++) atomicAdd(&cnt[(fv << tm.lsst) | fu], 1u);
+    }
+    __syncthreads();
+    for (int f = tid; f < nf; f += 256) cnt2[b * nf + f] = cnt[f];
+}
+
+// one workgroup per segment: per fine tile, running sums over the segment's
+// blocks (replacing the counts by block bases relative to the segment's first
+// entry), then the exclusive scan over fine tiles gives the bucket offsets
+__global__ __launch_bounds__(256) void k_l2_scan(TileMap tm, const long long* eoff1,
+                                                 const long long* boff, unsigned* cnt2,
+                                                 long long* offsets)
+{
+    __shared__ unsigned tot[kMaxFine];
     const int tid = threadIdx.x;
     const int nf = tm.sst * tm.sst;
     const long long k1 = blockIdx.x;
-    const long long beg = off1[k1], end = off1[k1 + 1];
-    for (int f = tid; f < nf; f += 256) cnt[f] = 0u;
-    __syncthreads();
-    for (long long r = beg + tid; r < end; r += 256) {
-        const int f = fine_of(rec, recwin, r);
-        atomicAdd(&cnt[f], 1u);
+    const long long b0 = boff[k1], b1 = boff[k1 + 1];
+    for (int f = tid; f < nf; f += 256) {
+        unsigned run = 0;
+        for (long long j = b0; j < b1; j++) {
+            const unsigned c = cnt2[j * nf + f];
+            cnt2[j * nf + f] = run;
+            run += c;
+        }
+        tot[f] = run;
     }
     __syncthreads();
     if (tid == 0) {
         unsigned run = 0;
         for (int f = 0; f < nf; f++) {
-            const unsigned c = cnt[f];
-            cnt[f] = run;
+            const unsigned c = tot[f];
+            tot[f] = run;
             run += c;
         }
     }
     __syncthreads();
-    for (int f = tid; f < nf; f += 256) offsets[k1 * nf + f] = beg + cnt[f];
-    if (k1 == 0 && tid == 0) offsets[tm.nk1 * nf] = off1[tm.nk1];
+    const long long beg = eoff1[k1];
+    for (int f = tid; f < nf; f += 256) {
+        offsets[k1 * nf + f] = beg + tot[f];
+        for (long long j = b0; j < b1; j++) cnt2[j * nf + f] += tot[f];
+    }
+    if (k1 == 0 && tid == 0) offsets[tm.nk1 * nf] = eoff1[tm.nk1];
+}
+
+template <int OUT, typename Rec, typename Out>
+__global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, const long long* eoff1,
+                                                    const long long* nblocks, const L2Block* blk,
+                                                    const unsigned* cnt2, const Rec* rec,
+                                                    Out* ent)
+{
+    __shared__ unsigned cur[kMaxFine];
+    const long long b = blockIdx.x;
+    if (b >= *nblocks) return;
+    const int tid = threadIdx.x;
+    const int nf = tm.sst * tm.sst;
+    for (int f = tid; f < nf; f += 256) cur[f] = cnt2[b * nf + f];
     __syncthreads();
-    for (long long r = beg + tid; r < end; r += 256) {
-        const int f = fine_of(rec, recwin, r);
-        const long long p = beg + atomicAdd(&cnt[f], 1u);
-        ent[p] = rec[r];
-        if constexpr (!HOME) entwin[p] = recwin[r] & 0xffffffu;
+    const L2Block d = blk[b];
+    int sv, su;
+    tm.super_of(d.k1, sv, su);
+    const long long beg = eoff1[d.k1];
+    for (long long r = d.beg + tid; r < d.end; r += 256) {
+        const Rec x = rec[r];
+        int fu_a, fu_b, fv_a, fv_b;
+        Window s;
+        if (!record_tiles<OUT>(x, g, tm, sv, su, fu_a, fu_b, fv_a, fv_b, s)) continue;
+        for (int fv = fv_a; fv <= fv_b; fv++) {
+            for (int fu = fu_a; fu <= fu_b; fu++) {
+                const long long p = beg + atomicAdd(&cur[(fv << tm.lsst) | fu], 1u);
+                if constexpr (OUT == kOutFp32) {
+                    const int u0 = ((su << tm.lsst) + fu) * kTile;
+                    const int v0 = ((sv << tm.lsst) + fv) * kTile;
+                    Entry32 e;
+                    e.wu = pack_cell(s.gu - u0, x.pu - (double)s.gu);
+                    e.wv = pack_cell(s.gv - v0, x.pv - (double)s.gv);
+                    e.vr = (float)x.vr;
+                    e.vi = (float)x.vi;
+                    ent[p] = e;
+                } else {
+                    ent[p] = x;
+                }
+            }
+        }
     }
 }
 
@@ -300,72 +418,82 @@ __global__ __launch_bounds__(256) void k_l2_sort(TileMap tm, const long long* of
 
 static inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
+static void scan(const unsigned* in, long long* out, long long n, long long* partials,
+                 long long* total_b, hipStream_t st)
+{
+    const long long nblk = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblk), dim3(256), 0, st, in, out, partials, n);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partials, nblk, out + n,
+                       total_b);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, st, out, partials, n);
+}
+
 hipError_t bucket_count(int mode, long long n, const double* uu, const double* vv, const double* ww,
                         const Geometry& g, const Buckets& b, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(b.map.nk1, 256)), dim3(256), 0, st, b.count1,
                        b.map.nk1, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(b.map.nk1, 256)), dim3(256), 0, st, b.count1e,
+                       b.map.nk1, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (unsigned*)b.err, 1LL, 0u);
     if (n > 0)
         hipLaunchKernelGGL(k_l1_count, dim3((unsigned)b.nwg), dim3(256),
-                           (size_t)b.map.nk1 * sizeof(unsigned), st, mode, n, uu, vv, ww, g, b.map,
-                           b.chunk, b.count1, b.base1, b.err);
+                           2 * (size_t)b.map.nk1 * sizeof(unsigned), st, mode, n, uu, vv, ww, g,
+                           b.map, b.chunk, b.count1, b.count1e, b.base1, b.err);
     return hipGetLastError();
 }
 
 hipError_t bucket_scan(const Buckets& b, hipStream_t st)
 {
-    const long long nb = b.map.nk1;
-    const long long nblk = (nb + kScanBlock - 1) / kScanBlock;
-    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblk), dim3(256), 0, st, b.count1, b.off1,
-                       b.partials, nb);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, b.partials, nblk, b.off1 + nb,
-                       b.total);
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(nb, 256)), dim3(256), 0, st, b.off1, b.partials,
-                       nb);
+    scan(b.count1, b.off1, b.map.nk1, b.partials, b.total_rec, st);
+    scan(b.count1e, b.eoff1, b.map.nk1, b.partials, b.total, st);
+    return hipGetLastError();
+}
+
+template <int OUT, typename Rec, typename Out>
+static hipError_t level2(const Geometry& g, const Buckets& b, const Rec* rec, Out* ent,
+                         hipStream_t st)
+{
+    const long long nk1 = b.map.nk1;
+    hipLaunchKernelGGL(k_l2_nblocks, dim3(blocks_for(nk1, 256)), dim3(256), 0, st, nk1, b.off1,
+                       b.l2_block, b.l2_nblk);
+    scan(b.l2_nblk, b.l2_boff, nk1, b.partials, b.l2_boff + nk1, st);
+    hipLaunchKernelGGL(k_l2_blocks, dim3(blocks_for(nk1, 256)), dim3(256), 0, st, nk1, b.off1,
+                       b.l2_block, b.l2_boff, b.l2_blk);
+    const unsigned grid = (unsigned)b.l2_max_blocks;
+    const long long* nblocks = b.l2_boff + nk1;
+    hipLaunchKernelGGL((k_l2_count<OUT, Rec>), dim3(grid), dim3(256), 0, st, g, b.map, nblocks,
+                       b.l2_blk, rec, b.l2_cnt);
+    hipLaunchKernelGGL(k_l2_scan, dim3((unsigned)nk1), dim3(256), 0, st, b.map, b.eoff1, b.l2_boff,
+                       b.l2_cnt, b.offsets);
+    hipLaunchKernelGGL((k_l2_scatter<OUT, Rec, Out>), dim3(grid), dim3(256), 0, st, g, b.map,
+                       b.eoff1, nblocks, b.l2_blk, b.l2_cnt, rec, ent);
     return hipGetLastError();
 }
 
 hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P,
                             const double* uu, const double* vv, const double* ww, const float* vr,
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
-                            void* rec, uint32_t* recwin, void* entries, uint32_t* wins,
-                            hipStream_t st)
+                            Entry64* rec, void* entries, hipStream_t st)
 {
-    const size_t lds = (size_t)b.map.nk1 * sizeof(unsigned);
-    if (n > 0) {
-        if (precision == LOFAR_FP64)
-            hipLaunchKernelGGL((k_l1_scatter_grid<true>), dim3((unsigned)b.nwg), dim3(256), lds, st,
-                               n, F, P, uu, vv, ww, vr, vi, wt, g, b.map, b.chunk, b.off1, b.base1,
-                               rec, recwin);
-        else
-            hipLaunchKernelGGL((k_l1_scatter_grid<false>), dim3((unsigned)b.nwg), dim3(256), lds,
-                               st, n, F, P, uu, vv, ww, vr, vi, wt, g, b.map, b.chunk, b.off1,
-                               b.base1, rec, recwin);
-    }
+    if (n > 0)
+        hipLaunchKernelGGL(k_l1_scatter_grid, dim3((unsigned)b.nwg), dim3(256),
+                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, F, P, uu, vv, ww, vr, vi,
+                           wt, g, b.map, b.chunk, b.off1, b.base1, rec);
     if (precision == LOFAR_FP64)
-        hipLaunchKernelGGL((k_l2_sort<Entry64, false>), dim3((unsigned)b.map.nk1), dim3(256), 0, st,
-                           b.map, b.off1, (const Entry64*)rec, recwin, (Entry64*)entries, wins,
-                           b.offsets);
-    else
-        hipLaunchKernelGGL((k_l2_sort<Entry32, false>), dim3((unsigned)b.map.nk1), dim3(256), 0, st,
-                           b.map, b.off1, (const Entry32*)rec, recwin, (Entry32*)entries, wins,
-                           b.offsets);
-    return hipGetLastError();
+        return level2<kOutFp64>(g, b, (const Entry64*)rec, (Entry64*)entries, st);
+    return level2<kOutFp32>(g, b, (const Entry64*)rec, (Entry32*)entries, st);
 }
 
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
                             const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
                             hipStream_t st)
 {
-    const size_t lds = (size_t)b.map.nk1 * sizeof(unsigned);
     if (n > 0)
-        hipLaunchKernelGGL(k_l1_scatter_home, dim3((unsigned)b.nwg), dim3(256), lds, st, n, uu, vv,
-                           ww, g, b.map, b.chunk, b.off1, b.base1, rec);
-    hipLaunchKernelGGL((k_l2_sort<DEntry, true>), dim3((unsigned)b.map.nk1), dim3(256), 0, st,
-                       b.map, b.off1, (const DEntry*)rec, (const uint32_t*)nullptr, entries,
-                       (uint32_t*)nullptr, b.offsets);
-    return hipGetLastError();
+        hipLaunchKernelGGL(k_l1_scatter_home, dim3((unsigned)b.nwg), dim3(256),
+                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, uu, vv, ww, g, b.map,
+                           b.chunk, b.off1, b.base1, rec);
+    return level2<kOutHome>(g, b, (const DEntry*)rec, entries, st);
 }
 
 }  // namespace lofar

@@ -205,39 +205,61 @@ Geometry make_geometry(int nw, int support, int nx, int ny, double dx, double dw
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Bucket-sort tuning: super-tile side (tiles) and samples per level-1
-// workgroup.  LOFAR_BUCKET_CFG="sst,chunk" overrides, for measurement.
-void bucket_cfg(int& sst, long long& chunk)
+// Bucket-sort tuning: super-tile side (tiles), samples per level-1 workgroup
+// and records per level-2 block.  LOFAR_BUCKET_CFG="sst,chunk,block" overrides,
+// for measurement.
+struct BucketCfg {
+    int sst;
+    long long chunk, block;
+};
+
+BucketCfg bucket_cfg()
 {
-    static int s_sst = 0;
-    static long long s_chunk = 0;
-    if (!s_sst) {
-        int a = 4;
-        long long b = 16384;
-        if (const char* e = getenv("LOFAR_BUCKET_CFG")) sscanf(e, "%d,%lld", &a, &b);
+    static BucketCfg cfg{0, 0, 0};
+    if (!cfg.sst) {
+        int a = 16;
+        long long b = 16384, l2 = 8192;
+        if (const char* e = getenv("LOFAR_BUCKET_CFG")) sscanf(e, "%d,%lld,%lld", &a, &b, &l2);
         if (a < 1 || a > 16 || (a & (a - 1))) a = 16;
         if (b < 256) b = 256;
-        s_chunk = b;
-        s_sst = a;
+        if (l2 < 256) l2 = 256;
+        cfg.chunk = b;
+        cfg.block = l2;
+        cfg.sst = a;
     }
-    sst = s_sst;
-    chunk = s_chunk;
+    return cfg;
+}
+
+// Level-2 block table and fine counts, placed after the records once the entry
+// total is known: returns the bytes needed; with base set, assigns the pointers.
+size_t level2_layout(Buckets& b, long long total, char* base)
+{
+    b.l2_max_blocks = (total + b.l2_block - 1) / b.l2_block + b.map.nk1;
+    const size_t nf = (size_t)b.map.sst * b.map.sst;
+    const size_t blk_bytes = align256((size_t)b.l2_max_blocks * sizeof(L2Block));
+    const size_t cnt_bytes = align256((size_t)b.l2_max_blocks * nf * sizeof(unsigned));
+    if (base) {
+        b.l2_blk = (L2Block*)base;
+        b.l2_cnt = (unsigned*)(base + blk_bytes);
+    }
+    return blk_bytes + cnt_bytes;
 }
 
 // Two-level counting sort of the samples into (w, tile) buckets: sizes the
 // workspace, counts per super-tile, scans, and reads back the entry total and
 // the validation flag (the only host synchronisation of a call).  The level-1
-// histogram of nw*ceil(ntv/sst)*ceil(ntu/sst) super-tiles lives in LDS and is
-// limited to kMaxSuperTiles (64 KiB).
-constexpr long long kMaxSuperTiles = 16384;
+// histograms of nw*ceil(ntv/sst)*ceil(ntu/sst) super-tiles (records and
+// entries) live in LDS and are limited to kMaxSuperTiles (64 KiB).
+constexpr long long kMaxSuperTiles = 8192;
 
 int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double* vv,
                   const double* ww, const Geometry& g, hipStream_t st, Buckets& b,
                   long long& total)
 {
-    int sst;
-    long long chunk;
-    bucket_cfg(sst, chunk);
+    const BucketCfg cfg = bucket_cfg();
+    int sst = cfg.sst;
+    const long long chunk = cfg.chunk;
+    b.l2_block = cfg.block;
     const int span = g.ntu > g.ntv ? g.ntu : g.ntv;
     while (sst > 1 && sst / 2 >= span) sst /= 2;  // no super-tile wider than the grid
     auto super_tiles = [&](int s) {
@@ -263,23 +285,33 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     b.chunk = n > 0 ? (n + b.nwg - 1) / b.nwg : 1;
     const long long nblk = (m.nk1 + 4095) / 4096;
     const size_t off_status = 0;
-    const size_t off_count1 = align256(off_status + 2 * sizeof(long long));
+    const size_t off_totrec = align256(off_status + 2 * sizeof(long long));
+    const size_t off_count1 = align256(off_totrec + sizeof(long long));
     const size_t off_off1 = align256(off_count1 + (size_t)m.nk1 * sizeof(unsigned));
-    const size_t off_base1 = align256(off_off1 + (size_t)(m.nk1 + 1) * sizeof(long long));
+    const size_t off_count1e = align256(off_off1 + (size_t)(m.nk1 + 1) * sizeof(long long));
+    const size_t off_eoff1 = align256(off_count1e + (size_t)m.nk1 * sizeof(unsigned));
+    const size_t off_base1 = align256(off_eoff1 + (size_t)(m.nk1 + 1) * sizeof(long long));
     const size_t off_offsets = align256(off_base1 + (size_t)(b.nwg * m.nk1) * sizeof(unsigned));
     const size_t off_partials =
         align256(off_offsets + (size_t)(m.nbuckets() + 1) * sizeof(long long));
-    const size_t bytes = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
+    const size_t off_l2nblk = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
+    const size_t off_l2boff = align256(off_l2nblk + (size_t)m.nk1 * sizeof(unsigned));
+    const size_t bytes = align256(off_l2boff + (size_t)(m.nk1 + 1) * sizeof(long long));
     int r = ensure(c, c.meta, bytes);
     if (r) return r;
     char* base = (char*)c.meta.ptr;
     b.total = (long long*)(base + off_status);
     b.err = (int*)(base + off_status + sizeof(long long));
+    b.total_rec = (long long*)(base + off_totrec);
     b.count1 = (unsigned*)(base + off_count1);
     b.off1 = (long long*)(base + off_off1);
+    b.count1e = (unsigned*)(base + off_count1e);
+    b.eoff1 = (long long*)(base + off_eoff1);
     b.base1 = (unsigned*)(base + off_base1);
     b.offsets = (long long*)(base + off_offsets);
     b.partials = (long long*)(base + off_partials);
+    b.l2_nblk = (unsigned*)(base + off_l2nblk);
+    b.l2_boff = (long long*)(base + off_l2boff);
     HIPCHK(stage(c, kStCount, st, [&] { return bucket_count(mode, n, uu, vv, ww, g, b, st); }));
     HIPCHK(stage(c, kStScan, st, [&] { return bucket_scan(b, st); }));
     HIPCHK(hipMemcpyAsync(c.status_host, b.total, 2 * sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -303,23 +335,23 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     long long total = 0;
     r = build_buckets(c, kBucketFootprint, n, uu, vv, ww, g, st, b, total);
     if (r) return r;
-    // level-1 records, their windows, the sorted entries and their windows
+    // level-1 records (at most one per entry), the sorted entries, then the
+    // level-2 block table
     const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
+    const size_t rbytes = align256((size_t)total * sizeof(Entry64));
     const size_t ebytes = align256((size_t)total * esz);
-    const size_t wbytes = align256((size_t)total * sizeof(uint32_t));
-    r = ensure(c, c.ents, 2 * (ebytes + wbytes) + 256);
+    const size_t l2 = level2_layout(b, total, nullptr);
+    r = ensure(c, c.ents, rbytes + ebytes + l2 + 256);
     if (r) return r;
     char* eb = (char*)c.ents.ptr;
-    void* rec = eb;
-    uint32_t* recwin = (uint32_t*)(eb + ebytes);
-    void* ents = eb + ebytes + wbytes;
-    uint32_t* wins = (uint32_t*)(eb + 2 * ebytes + wbytes);
+    Entry64* rec = (Entry64*)eb;
+    void* ents = eb + rbytes;
+    level2_layout(b, total, eb + rbytes + ebytes);
     HIPCHK(stage(c, kStScatter, st, [&] {
-        return bucket_sort_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, rec, recwin,
-                                ents, wins, st);
+        return bucket_sort_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, rec, ents, st);
     }));
     HIPCHK(stage(c, kStGrid, st,
-                 [&] { return grid_tiles(precision, g, b, ents, wins, cube, accumulate, st); }));
+                 [&] { return grid_tiles(precision, g, b, ents, cube, accumulate, st); }));
     return LOFAR_OK;
 }
 
@@ -578,10 +610,14 @@ int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long
     long long total = 0;
     r = build_buckets(c, kBucketHome, num_points, uu, vv, ww, g, cg.st, b, total);
     if (r) return r;
-    r = ensure(c, c.ents, 2 * (size_t)total * sizeof(DEntry) + 256);
+    const size_t ebytes = align256((size_t)total * sizeof(DEntry));
+    const size_t l2 = level2_layout(b, total, nullptr);
+    r = ensure(c, c.ents, 2 * ebytes + l2 + 256);
     if (r) return r;
-    DEntry* rec = (DEntry*)c.ents.ptr;
-    DEntry* ents = rec + total;
+    char* eb = (char*)c.ents.ptr;
+    DEntry* rec = (DEntry*)eb;
+    DEntry* ents = (DEntry*)(eb + ebytes);
+    level2_layout(b, total, eb + 2 * ebytes);
     HIPCHK(stage(c, kStScatter, cg.st, [&] {
         return bucket_sort_home(num_points, uu, vv, ww, g, b, rec, ents, cg.st);
     }));

@@ -27,31 +27,39 @@ struct Geometry {
     double dx, dw;      // cell size in normalised uv units, plane width in w units
 };
 
-// Packs a per-tile clipped window [a0,a1] x [b0,b1] (tile-local, each < 64)
-// and, in the top byte, the tile's index inside its super-tile (bucket sort).
-__host__ __device__ inline uint32_t pack_win(int a0, int a1, int b0, int b1, int fine = 0)
-{
-    return (uint32_t)a0 | ((uint32_t)a1 << 6) | ((uint32_t)b0 << 12) | ((uint32_t)b1 << 18) |
-           ((uint32_t)fine << 24);
-}
-
-// Gridding entry, FP64 mode: absolute grid position (uu/dx, vv/dx) and the
-// visibility summed over channels and polarisations with the reference's
-// weighting (w-stacking_omp.c:118-133).
-struct alignas(16) Entry64 {
+// Gridding entry, FP64 mode (and the level-1 record of both modes): absolute
+// grid position (uu/dx, vv/dx) and the visibility summed over channels and
+// polarisations with the reference's weighting (w-stacking_omp.c:118-133).
+// The gridder derives the window and its clipping to the tile from the position.
+struct alignas(32) Entry64 {
     double pu, pv;
     double vr, vi;
 };
 
-// Gridding entry, FP32 mode: distances from the first window cell of the tile
-// (j0+0.5-pos_u, k0+0.5-pos_v) and the weighted visibility.
+// Gridding entry, FP32 mode: per axis, the sample's cell relative to the tile
+// origin (gu-u0+16, 8 bits) and its offset within the cell (pu-gu+1 in
+// (0, 2), 23 fraction bits) packed in one word; the weighted visibility as float.
 struct alignas(16) Entry32 {
-    float du0, dv0;
+    unsigned wu, wv;
     float vr, vi;
 };
 
+constexpr int kCellBias = 16;  // gu-u0 lies in [-K, 63+K] for every entry of a tile, K <= 8
+
+__host__ __device__ inline unsigned pack_cell(int cell_rel, double frac)
+{
+    double x = (frac + 1.0) * 8388608.0;  // 2^23
+    x = x < 0.0 ? 0.0 : (x > 16777215.0 ? 16777215.0 : x);
+    return ((unsigned)(cell_rel + kCellBias) << 24) | (unsigned)(long long)(x + 0.5);
+}
+__host__ __device__ inline int cell_of(unsigned w) { return (int)(w >> 24) - kCellBias; }
+__host__ __device__ inline float frac_of(unsigned w)
+{
+    return (float)(w & 0xffffffu) * 0x1p-23f - 1.0f;
+}
+
 // Degridding entry: position and the sample's index in the caller's order.
-struct alignas(16) DEntry {
+struct alignas(32) DEntry {
     double pu, pv;
     long long idx;
     long long pad;

@@ -18,14 +18,54 @@ namespace lofar {
 // ---------------------------------------------------------------------------
 // gridding
 
+// Window of an entry at truncated cell (gu, gv), clipped to the grid as
+// w-stacking_omp.c:93-96 does and then to the tile at (u0, v0): first cell
+// (a0, b0) and inclusive extents (na, nb), tile-local.
+struct TileWin {
+    int a0, na, b0, nb;
+};
+__device__ __forceinline__ TileWin clip_window(int gu, int gv, const Geometry& g, int u0, int v0)
+{
+    const int j0 = (gu > g.K - 1) ? gu - g.K : 0;
+    const int j1 = (gu < g.nx - g.K) ? gu + g.K : g.nx - 1;
+    const int k0 = (gv > g.K - 1) ? gv - g.K : 0;
+    const int k1 = (gv < g.ny - g.K) ? gv + g.K : g.ny - 1;
+    TileWin t;
+    t.a0 = max(j0, u0) - u0;
+    t.na = min(j1, u0 + kTile - 1) - u0 - t.a0;
+    t.b0 = max(k0, v0) - v0;
+    t.nb = min(k1, v0 + kTile - 1) - v0 - t.b0;
+    return t;
+}
+
+// FP32 entry decode: the window in the tile, and the centre of the window's
+// first cell relative to the sample (j0 + 0.5 - pu) as an exact integer part
+// plus the cell offset (0.5 - (pu - gu)).
+struct Win32 {
+    TileWin t;
+    int iu, iv;      // u0 + a0 - gu, v0 + b0 - gv
+    float hu, hv;    // 0.5 - (pu - gu), 0.5 - (pv - gv), exact in float
+};
+__device__ __forceinline__ Win32 decode32(const Entry32& en, const Geometry& g, int u0, int v0)
+{
+    Win32 w;
+    const int gu = u0 + cell_of(en.wu), gv = v0 + cell_of(en.wv);
+    w.t = clip_window(gu, gv, g, u0, v0);
+    w.iu = u0 + w.t.a0 - gu;
+    w.iv = v0 + w.t.b0 - gv;
+    w.hu = 0.5f - frac_of(en.wu);
+    w.hv = 0.5f - frac_of(en.wv);
+    return w;
+}
+
 // One workgroup per (w, tile_v, tile_u) bucket, one lane per entry.  The
 // Gaussian weight norm*exp(-(du^2+dv^2)/2) (w-stacking_omp.c:13-18,50-52) is
 // evaluated as the separable product norm*exp(-du^2/2)*exp(-dv^2/2): 2*WIN
 // exponentials per entry instead of WIN^2.
-template <int WIN, bool FP64>
-__global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, const long long* offsets,
-                                                    const void* entries, const uint32_t* wins,
-                                                    void* cube, int accumulate)
+template <int WIN, bool FP64, int NT>
+__global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const long long* offsets,
+                                                    const void* entries, void* cube,
+                                                    int accumulate)
 {
     __shared__ double acc[2 * kTile * kTile];
     const long long bkt = blockIdx.x;
@@ -40,7 +80,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, cons
 
     if (beg == end) {
         if (!accumulate) {
-            for (int c = tid; c < kTile * kTile; c += 256) {
+            for (int c = tid; c < kTile * kTile; c += NT) {
                 const int row = c / kTile, col = c % kTile;
                 if (row < trows && col < tcols) {
                     const long long o = cell0 + (long long)row * g.nx + col;
@@ -53,7 +93,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, cons
         }
         return;
     }
-    for (int c = tid; c < kTile * kTile; c += 256) {
+    for (int c = tid; c < kTile * kTile; c += NT) {
         const int row = c / kTile, col = c % kTile;
         double re = 0.0, im = 0.0;
         if (accumulate && row < trows && col < tcols) {
@@ -74,13 +114,12 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, cons
     __syncthreads();
 
     const double norm = 0.5 / LOFAR_REF_PI;  // std22/PI with std = 1 cell
-    for (long long e = beg + tid; e < end; e += 256) {
-        const uint32_t wn = wins[e];
-        const int a0 = wn & 63, na = (int)((wn >> 6) & 63) - a0;
-        const int b0 = (wn >> 12) & 63, nb = (int)((wn >> 18) & 63) - b0;
-        double* base = &acc[2 * (b0 * kTile + a0)];
+    for (long long e = beg + tid; e < end; e += NT) {
         if constexpr (FP64) {
             const Entry64 en = ((const Entry64*)entries)[e];
+            const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
+            const int a0 = tw.a0, na = tw.na, b0 = tw.b0, nb = tw.nb;
+            double* base = &acc[2 * (b0 * kTile + a0)];
             double eu[WIN];
 #pragma unroll
             for (int q = 0; q < WIN; q++) {
@@ -105,16 +144,19 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, cons
             }
         } else {
             const Entry32 en = ((const Entry32*)entries)[e];
+            const Win32 w32 = decode32(en, g, u0, v0);
+            const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
+            double* base = &acc[2 * (b0 * kTile + a0)];
             float eu[WIN];
 #pragma unroll
             for (int q = 0; q < WIN; q++) {
-                const float du = en.du0 + (float)q;
+                const float du = (float)(w32.iu + q) + w32.hu;
                 eu[q] = (q <= na) ? expf(-(du * du) * 0.5f) : 0.0f;
             }
 #pragma unroll
             for (int kq = 0; kq < WIN; kq++) {
                 if (kq <= nb) {
-                    const float dv = en.dv0 + (float)kq;
+                    const float dv = (float)(w32.iv + kq) + w32.hv;
                     const float f = (float)norm * expf(-(dv * dv) * 0.5f);
                     const float fr = f * en.vr, fi = f * en.vi;
                     double* row = base + 2 * kq * kTile;
@@ -130,7 +172,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, cons
         }
     }
     __syncthreads();
-    for (int c = tid; c < kTile * kTile; c += 256) {
+    for (int c = tid; c < kTile * kTile; c += NT) {
         const int row = c / kTile, col = c % kTile;
         if (row < trows && col < tcols) {
             const long long o = cell0 + (long long)row * g.nx + col;
@@ -144,30 +186,38 @@ __global__ __launch_bounds__(256) void k_grid_tiles(Geometry g, TileMap tm, cons
 
 // FP32C gridding with exact fixed-point accumulation.  Every contribution
 // eu*(norm*ev*vis) is formed exactly as the double product of two floats,
-// scaled by a per-tile power of two 2^S and rounded down to a 64-bit integer;
-// the LDS tile accumulates those integers with ds_add_u64 (about twice the rate
-// of ds_add_f64 and order-independent, hence bitwise reproducible).  S leaves
-// log2(entries)+2 bits of headroom below 2^61 for the largest possible cell sum
-// (every entry of the tile at its maximum |vis|*norm), so no sum can overflow,
-// and the quantum is ~2^-(59-log2(entries)) of the tile's largest contribution.
-// The flush converts each integer sum back to double (times 2^-S) and rounds
-// once to float.  A tile holding a non-finite visibility takes the FP64-atomic
-// path of k_grid_tiles instead (the reference propagates inf/NaN into the grid).
+// scaled by a per-tile power of two 2^S and rounded to the nearest 64-bit
+// integer; the LDS tile accumulates those integers with ds_add_u64 (about twice
+// the rate of ds_add_f64 and order-independent, hence bitwise reproducible).
+// S leaves log2(entries)+2 bits of headroom below 2^51 for the largest possible
+// cell sum (every entry of the tile at its maximum |vis|*norm), so no sum can
+// overflow, and the quantum is ~2^-(49-log2(entries)) of the tile's largest
+// contribution (far below the float rounding of the output).  The flush
+// converts each integer sum back to double (exact below 2^53) times 2^-S and
+// rounds once to float.  A tile holding a non-finite visibility takes the
+// FP64-atomic path of k_grid_tiles instead (the reference propagates inf/NaN
+// into the grid).
+constexpr int kFixedBits = 51;
+
+// Round |y| < 2^51 to the nearest integer as a two's-complement 64-bit value:
+// adding 1.5*2^52 leaves round(y) in the low 52 bits of the sum's encoding,
+// whose low word is 0 for the constant itself, so only the high word needs the
+// constant's high word (0x43380000) taken off.
 __device__ __forceinline__ unsigned long long to_fixed(double y)
 {
-    // y = hi*2^32 + lo with 0 <= lo < 2^32, |y| < 2^61
-    const double hi = floor(y * 0x1p-32);
-    const double lo = fma(-hi, 0x1p32, y);
-    return ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)(unsigned)lo;
+    const double t = y + 0x1.8p52;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(t);
+    const unsigned lo = (unsigned)b;
+    const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
+    return ((unsigned long long)hi << 32) | lo;
 }
 
-template <int WIN>
-__global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, const long long* offsets,
-                                                       const Entry32* entries,
-                                                       const uint32_t* wins, float2* cube)
+template <int WIN, int NT>
+__global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, const long long* offsets,
+                                                       const Entry32* entries, float2* cube)
 {
     __shared__ unsigned long long acc[2 * kTile * kTile];
-    __shared__ float wred[4];
+    __shared__ float wred[NT / 64];
     const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
@@ -180,7 +230,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, c
     const double norm = 0.5 / LOFAR_REF_PI;
 
     if (beg == end) {
-        for (int c = tid; c < kTile * kTile; c += 256) {
+        for (int c = tid; c < kTile * kTile; c += NT) {
             const int row = c / kTile, col = c % kTile;
             if (row < trows && col < tcols)
                 cube[cell0 + (long long)row * g.nx + col] = float2{0.0f, 0.0f};
@@ -191,7 +241,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, c
     // non-finite value sends the tile down the floating-point path)
     float m = 0.0f;
     bool finite = true;
-    for (long long e = beg + tid; e < end; e += 256) {
+    for (long long e = beg + tid; e < end; e += NT) {
         const Entry32 en = entries[e];
         finite = finite && isfinite(en.vr) && isfinite(en.vi);
         m = fmaxf(m, fmaxf(fabsf(en.vr), fabsf(en.vi)));
@@ -203,28 +253,29 @@ __global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, c
     if (tid == 0) all_finite = 1;
     __syncthreads();
     if (!finite) all_finite = 0;
-    m = fmaxf(fmaxf(wred[0], wred[1]), fmaxf(wred[2], wred[3]));
-    for (int c = tid; c < 2 * kTile * kTile; c += 256) acc[c] = 0ull;  // also 0.0 as double
+    m = wred[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; i++) m = fmaxf(m, wred[i]);
+    for (int c = tid; c < 2 * kTile * kTile; c += NT) acc[c] = 0ull;  // also 0.0 as double
     __syncthreads();
     if (!all_finite) {
         // floating-point accumulation (ds_add_f64), as k_grid_tiles<WIN, false>
         double* dacc = (double*)acc;
-        for (long long e = beg + tid; e < end; e += 256) {
-            const uint32_t wn = wins[e];
-            const int a0 = wn & 63, na = (int)((wn >> 6) & 63) - a0;
-            const int b0 = (wn >> 12) & 63, nb = (int)((wn >> 18) & 63) - b0;
-            double* base = &dacc[2 * (b0 * kTile + a0)];
+        for (long long e = beg + tid; e < end; e += NT) {
             const Entry32 en = entries[e];
+            const Win32 w32 = decode32(en, g, u0, v0);
+            const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
+            double* base = &dacc[2 * (b0 * kTile + a0)];
             float eu[WIN];
 #pragma unroll
             for (int q = 0; q < WIN; q++) {
-                const float du = en.du0 + (float)q;
+                const float du = (float)(w32.iu + q) + w32.hu;
                 eu[q] = (q <= na) ? expf(-(du * du) * 0.5f) : 0.0f;
             }
 #pragma unroll
             for (int kq = 0; kq < WIN; kq++) {
                 if (kq <= nb) {
-                    const float dv = en.dv0 + (float)kq;
+                    const float dv = (float)(w32.iv + kq) + w32.hv;
                     const float f = (float)norm * expf(-(dv * dv) * 0.5f);
                     const float fr = f * en.vr, fi = f * en.vi;
                     double* row = base + 2 * kq * kTile;
@@ -239,7 +290,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, c
             }
         }
         __syncthreads();
-        for (int c = tid; c < kTile * kTile; c += 256) {
+        for (int c = tid; c < kTile * kTile; c += NT) {
             const int row = c / kTile, col = c % kTile;
             if (row < trows && col < tcols)
                 cube[cell0 + (long long)row * g.nx + col] =
@@ -251,25 +302,24 @@ __global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, c
     frexp(norm * (double)(m > 0.0f ? m : 1.0f), &ex);  // norm*m < 2^ex
     int hb = 2;
     for (long long cnt = end - beg; cnt > 1; cnt >>= 1) hb++;
-    const int S = 61 - hb - ex;
+    const int S = kFixedBits - hb - ex;
     const double scale = ldexp(1.0, S);
 
-    for (long long e = beg + tid; e < end; e += 256) {
-        const uint32_t wn = wins[e];
-        const int a0 = wn & 63, na = (int)((wn >> 6) & 63) - a0;
-        const int b0 = (wn >> 12) & 63, nb = (int)((wn >> 18) & 63) - b0;
-        unsigned long long* base = &acc[2 * (b0 * kTile + a0)];
+    for (long long e = beg + tid; e < end; e += NT) {
         const Entry32 en = entries[e];
+        const Win32 w32 = decode32(en, g, u0, v0);
+        const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
+        unsigned long long* base = &acc[2 * (b0 * kTile + a0)];
         double eu[WIN];
 #pragma unroll
         for (int q = 0; q < WIN; q++) {
-            const float du = en.du0 + (float)q;
+            const float du = (float)(w32.iu + q) + w32.hu;
             eu[q] = (q <= na) ? (double)expf(-(du * du) * 0.5f) : 0.0;
         }
 #pragma unroll
         for (int kq = 0; kq < WIN; kq++) {
             if (kq <= nb) {
-                const float dv = en.dv0 + (float)kq;
+                const float dv = (float)(w32.iv + kq) + w32.hv;
                 const float f = (float)norm * expf(-(dv * dv) * 0.5f);
                 const double fr = (double)(f * en.vr) * scale, fi = (double)(f * en.vi) * scale;
                 unsigned long long* row = base + 2 * kq * kTile;
@@ -285,7 +335,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles_fx(Geometry g, TileMap tm, c
     }
     __syncthreads();
     const double inv = ldexp(1.0, -S);
-    for (int c = tid; c < kTile * kTile; c += 256) {
+    for (int c = tid; c < kTile * kTile; c += NT) {
         const int row = c / kTile, col = c % kTile;
         if (row < trows && col < tcols) {
             const double re = (double)(long long)acc[2 * c] * inv;
@@ -401,31 +451,55 @@ static int grid_cfg()
     return cfg;
 }
 
+// threads per gridding workgroup: LOFAR_GRID_NT (256, 512 or 1024), for measurement
+static int grid_nt()
+{
+    static int nt = -1;
+    if (nt < 0) {
+        const char* s = getenv("LOFAR_GRID_NT");
+        nt = s ? atoi(s) : 512;
+        if (nt != 256 && nt != 1024) nt = 512;
+    }
+    return nt;
+}
+
+template <int WIN, int NT>
+static void launch_grid_nt(int precision, const Geometry& g, const Buckets& b, const void* entries,
+                           void* cube, int accumulate, hipStream_t st)
+{
+    const dim3 grid((unsigned)b.map.nbuckets());
+    if (precision == LOFAR_FP64)
+        hipLaunchKernelGGL((k_grid_tiles<WIN, true, NT>), grid, dim3(NT), 0, st, g, b.map,
+                           b.offsets, entries, cube, accumulate);
+    else if (!accumulate && grid_cfg() == 0)
+        hipLaunchKernelGGL((k_grid_tiles_fx<WIN, NT>), grid, dim3(NT), 0, st, g, b.map, b.offsets,
+                           (const Entry32*)entries, (float2*)cube);
+    else
+        hipLaunchKernelGGL((k_grid_tiles<WIN, false, NT>), grid, dim3(NT), 0, st, g, b.map,
+                           b.offsets, entries, cube, accumulate);
+}
+
 template <int WIN>
 static void launch_grid(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                        const uint32_t* wins, void* cube, int accumulate, hipStream_t st)
+                        void* cube, int accumulate, hipStream_t st)
 {
-    if (precision == LOFAR_FP64)
-        hipLaunchKernelGGL((k_grid_tiles<WIN, true>), dim3((unsigned)b.map.nbuckets()), dim3(256), 0, st, g,
-                           b.map, b.offsets, entries, wins, cube, accumulate);
-    else if (!accumulate && grid_cfg() == 0)
-        hipLaunchKernelGGL((k_grid_tiles_fx<WIN>), dim3((unsigned)b.map.nbuckets()), dim3(256), 0, st, g,
-                           b.map, b.offsets, (const Entry32*)entries, wins, (float2*)cube);
-    else
-        hipLaunchKernelGGL((k_grid_tiles<WIN, false>), dim3((unsigned)b.map.nbuckets()), dim3(256), 0, st, g,
-                           b.map, b.offsets, entries, wins, cube, accumulate);
+    switch (grid_nt()) {
+        case 1024: return launch_grid_nt<WIN, 1024>(precision, g, b, entries, cube, accumulate, st);
+        case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, st);
+        default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, st);
+    }
 }
 
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                      const uint32_t* wins, void* cube, int accumulate, hipStream_t st)
+                      void* cube, int accumulate, hipStream_t st)
 {
     const int win = 2 * g.K + 1;
     if (win <= 7)
-        launch_grid<7>(precision, g, b, entries, wins, cube, accumulate, st);
+        launch_grid<7>(precision, g, b, entries, cube, accumulate, st);
     else if (win <= 9)
-        launch_grid<9>(precision, g, b, entries, wins, cube, accumulate, st);
+        launch_grid<9>(precision, g, b, entries, cube, accumulate, st);
     else if (win <= 17)
-        launch_grid<17>(precision, g, b, entries, wins, cube, accumulate, st);
+        launch_grid<17>(precision, g, b, entries, cube, accumulate, st);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -4,11 +4,30 @@
 
 namespace lofar {
 
-// Geometry of one sample, exactly as w-stacking_omp.c:85-96 derives it.
-struct SampleWin {
-    double pu, pv;       // uu/dx, vv/dx
-    int gw, gu, gv;      // (int) truncations
-    int j0, j1, k0, k1;  // clipped window (may be empty)
+// Window of a sample at grid position (pu, pv) = (uu/dx, vv/dx), exactly as
+// w-stacking_omp.c:85-96 derives it: (int) truncations and the clipping rule
+// at the grid edges.  Empty when j0 > j1 or k0 > k1.
+struct Window {
+    int gu, gv;          // (int) truncations
+    int j0, j1, k0, k1;  // clipped window
+};
+
+__host__ __device__ inline Window window_of(double pu, double pv, const Geometry& g)
+{
+    Window s;
+    s.gu = (int)pu;
+    s.gv = (int)pv;
+    s.j0 = (s.gu > g.K - 1) ? s.gu - g.K : 0;
+    s.j1 = (s.gu < g.nx - g.K) ? s.gu + g.K : g.nx - 1;
+    s.k0 = (s.gv > g.K - 1) ? s.gv - g.K : 0;
+    s.k1 = (s.gv < g.ny - g.K) ? s.gv + g.K : g.ny - 1;
+    return s;
+}
+
+// Validated geometry of one sample.
+struct SampleWin : Window {
+    double pu, pv;  // uu/dx, vv/dx
+    int gw;         // (int)(ww/dw)
 };
 
 __device__ __forceinline__ bool sample_window(double u, double v, double w, const Geometry& g,
@@ -19,16 +38,18 @@ __device__ __forceinline__ bool sample_window(double u, double v, double w, cons
     const double lim = 1073741824.0;  // 2^30 keeps every (int) cast defined
     if (!(fabs(pu) < lim) || !(fabs(pv) < lim)) return false;
     if (!(pw > -1.0) || !(pw < (double)g.nw)) return false;
+    static_cast<Window&>(s) = window_of(pu, pv, g);
     s.pu = pu;
     s.pv = pv;
     s.gw = (int)pw;
-    s.gu = (int)pu;
-    s.gv = (int)pv;
-    s.j0 = (s.gu > g.K - 1) ? s.gu - g.K : 0;
-    s.j1 = (s.gu < g.nx - g.K) ? s.gu + g.K : g.nx - 1;
-    s.k0 = (s.gv > g.K - 1) ? s.gv - g.K : 0;
-    s.k1 = (s.gv < g.ny - g.K) ? s.gv + g.K : g.ny - 1;
     return true;
+}
+
+// Home cell of a sample for degridding: its truncated position clamped to the grid.
+__device__ __forceinline__ void home_cell(const Window& s, const Geometry& g, int& cu, int& cv)
+{
+    cu = min(max(s.gu, 0), g.nx - 1);
+    cv = min(max(s.gv, 0), g.ny - 1);
 }
 
 // Bucket order of the (w-plane, tile) buckets: super-tiles of sst x sst tiles,
@@ -40,6 +61,16 @@ struct TileMap {
     __host__ __device__ long long key1(int w, int tv, int tu) const
     {
         return ((long long)w * nstv + (tv >> lsst)) * nstu + (tu >> lsst);
+    }
+    __host__ __device__ long long key_st(int w, int sv, int su) const
+    {
+        return ((long long)w * nstv + sv) * nstu + su;
+    }
+    // super-tile coordinates of level-1 key k1
+    __host__ __device__ void super_of(long long k1, int& sv, int& su) const
+    {
+        su = (int)(k1 % nstu);
+        sv = (int)((k1 / nstu) % nstv);
     }
     __host__ __device__ int fine(int tv, int tu) const
     {
@@ -61,36 +92,53 @@ struct TileMap {
     }
 };
 
-// Workspace of one bucketing of n samples (two-level counting sort).
+// A level-2 block: records [beg, end) of super-tile segment k1.
+struct alignas(32) L2Block {
+    long long k1, beg, end, pad;
+};
+
+// Workspace of one bucketing of n samples (two-level counting sort).  Level 1
+// writes one record per (sample, super-tile its window touches), level 2
+// expands every record into one entry per tile of its window in the super-tile.
 struct Buckets {
     TileMap map;
     long long nwg, chunk;  // level-1 workgroups and samples per workgroup
-    unsigned* count1;      // [nk1] entries per super-tile
-    long long* off1;       // [nk1+1]
+    unsigned* count1;      // [nk1] records per super-tile
+    long long* off1;       // [nk1+1] record offsets
+    unsigned* count1e;     // [nk1] entries per super-tile
+    long long* eoff1;      // [nk1+1] entry offsets
     unsigned* base1;       // [nwg*nk1] each workgroup's first slot in each super-tile
     long long* offsets;    // [nbuckets+1] entries of every (w, tile) bucket, bucket order
     long long* partials;   // scan partials
     long long* total;      // host-visible total entry count (status[0])
+    long long* total_rec;  // total record count (device)
     int* err;              // host-visible validation flag (status[1])
+    // level 2 (set once the entry total is known)
+    long long l2_block;       // records per level-2 block
+    long long l2_max_blocks;  // grid of the per-block kernels: ceil(total/l2_block) + nk1
+    unsigned* l2_nblk;        // [nk1] blocks per segment
+    long long* l2_boff;       // [nk1+1] first block of each segment; [nk1] = block count
+    L2Block* l2_blk;          // [l2_max_blocks]
+    unsigned* l2_cnt;         // [l2_max_blocks * sst^2] fine counts, then block bases
 };
 
 enum { kBucketFootprint = 0, kBucketHome = 1 };
 
-// Level 1 (bucket.hip): count the (sample, tile) pairs per super-tile.
+// Level 1 (bucket.hip): count records and entries per super-tile, validate.
 hipError_t bucket_count(int mode, long long n, const double* uu, const double* vv, const double* ww,
                         const Geometry& g, const Buckets& b, hipStream_t st);
 hipError_t bucket_scan(const Buckets& b, hipStream_t st);
-// Level 1 scatter into super-tile order, then level 2 into (w, tile) order.
+// Level 1 scatter into super-tile order (rec: one Entry64 per record), then level
+// 2 into (w, tile) order (entries: Entry64 for FP64, Entry32 for FP32C).
 hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P,
                             const double* uu, const double* vv, const double* ww, const float* vr,
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
-                            void* rec, uint32_t* recwin, void* entries, uint32_t* wins,
-                            hipStream_t st);
+                            Entry64* rec, void* entries, hipStream_t st);
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
                             const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
                             hipStream_t st);
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                      const uint32_t* wins, void* cube, int accumulate, hipStream_t st);
+                      void* cube, int accumulate, hipStream_t st);
 hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, const DEntry* entries,
                         const void* cube, const float* weight, long long F, long long P,
                         void* model_re, void* model_im, hipStream_t st);

@@ -143,9 +143,9 @@ print("BUCKETS_OK")
 """
 
 
-@pytest.mark.parametrize("cfg", ["1,256", "2,1000", "16,300", "8,1000000"])
+@pytest.mark.parametrize("cfg", ["1,256,256", "2,1000,300", "16,300,100000", "8,1000000,8192"])
 def test_bucket_sort_configurations(cuda, cfg):
-    """Other super-tile sides and level-1 chunk sizes (LOFAR_BUCKET_CFG, read once per
+    """Other super-tile sides, level-1 chunk and level-2 block sizes (LOFAR_BUCKET_CFG, read once per
     process) give the same cube; run in a child process with the setting applied."""
     import subprocess
     import sys
