@@ -24,6 +24,7 @@ namespace lofar {
 
 constexpr int kScanBlock = 4096;  // elements per block of the local scan (256 threads x 16)
 constexpr int kMaxFine = 256;     // sst <= 16
+constexpr int kUnroll = 4;        // samples or records per thread per round
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, long long n, unsigned v)
 {
@@ -56,7 +57,11 @@ __global__ __launch_bounds__(256) void k_l1_count(int mode, long long n, const d
     const long long beg = (long long)blockIdx.x * chunk;
     const long long end = min(n, beg + chunk);
     bool bad = false;
-    for (long long i = beg + tid; i < end; i += 256) {
+    for (long long i0 = beg + tid; i0 < end; i0 += 256 * kUnroll) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const long long i = i0 + (long long)u * 256;
+        if (i >= end) break;
         SampleWin s;
         if (!sample_window(uu[i], vv[i], ww[i], g, s)) {
             bad = true;
@@ -82,6 +87,7 @@ __global__ __launch_bounds__(256) void k_l1_count(int mode, long long n, const d
                 atomicAdd(&hent[k], (unsigned)(nu * nv));
             }
         }
+      }
     }
     if (bad) atomicOr(err, 1);
     __syncthreads();
@@ -205,20 +211,30 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
     __syncthreads();
     const long long beg = (long long)blockIdx.x * chunk;
     const long long end = min(n, beg + chunk);
-    for (long long i = beg + tid; i < end; i += 256) {
-        SampleWin s;
-        if (!sample_window(uu[i], vv[i], ww[i], g, s)) continue;
-        if (s.j0 > s.j1 || s.k0 > s.k1) continue;
-        Entry64 e;
-        e.pu = s.pu;
-        e.pv = s.pv;
-        sample_vis(i, F, P, vr, vi, wt, e.vr, e.vi);
-        const TileSpan t = tiles_of(s);
-        for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++)
-            for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
-                const long long k = tm.key_st(s.gw, sv, su);
-                rec[off1[k] + atomicAdd(&cur[k], 1u)] = e;
-            }
+    // kUnroll samples per thread per round, their loads issued together
+    for (long long i0 = beg + tid; i0 < end; i0 += 256 * kUnroll) {
+        SampleWin s[kUnroll];
+        Entry64 e[kUnroll];
+        bool ok[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const long long i = min(i0 + (long long)u * 256, end - 1);
+            ok[u] = i0 + (long long)u * 256 < end && sample_window(uu[i], vv[i], ww[i], g, s[u]) &&
+                    s[u].j0 <= s[u].j1 && s[u].k0 <= s[u].k1;
+            sample_vis(i, F, P, vr, vi, wt, e[u].vr, e[u].vi);
+            e[u].pu = s[u].pu;
+            e[u].pv = s[u].pv;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (!ok[u]) continue;
+            const TileSpan t = tiles_of(s[u]);
+            for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++)
+                for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
+                    const long long k = tm.key_st(s[u].gw, sv, su);
+                    rec[off1[k] + atomicAdd(&cur[k], 1u)] = e[u];
+                }
+        }
     }
 }
 
@@ -321,12 +337,19 @@ __global__ __launch_bounds__(256) void k_l2_count(Geometry g, TileMap tm, const 
     const L2Block d = blk[b];
     int sv, su;
     tm.super_of(d.k1, sv, su);
-    for (long long r = d.beg + tid; r < d.end; r += 256) {
-        int fu_a, fu_b, fv_a, fv_b;
-        Window s;
-        if (!record_tiles<OUT>(rec[r], g, tm, sv, su, fu_a, fu_b, fv_a, fv_b, s)) continue;
-        for (int fv = fv_a; fv <= fv_b; fv++)
-            for (int fu = fu_a; fu <= fu_b; fu++) atomicAdd(&cnt[(fv << tm.lsst) | fu], 1u);
+    for (long long r0 = d.beg + tid; r0 < d.end; r0 += 256 * kUnroll) {
+        Rec xs[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) xs[u] = rec[min(r0 + (long long)u * 256, d.end - 1)];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (r0 + (long long)u * 256 >= d.end) break;
+            int fu_a, fu_b, fv_a, fv_b;
+            Window s;
+            if (!record_tiles<OUT>(xs[u], g, tm, sv, su, fu_a, fu_b, fv_a, fv_b, s)) continue;
+            for (int fv = fv_a; fv <= fv_b; fv++)
+                for (int fu = fu_a; fu <= fu_b; fu++) atomicAdd(&cnt[(fv << tm.lsst) | fu], 1u);
+        }
     }
     __syncthreads();
     for (int f = tid; f < nf; f += 256) cnt2[b * nf + f] = cnt[f];
@@ -388,8 +411,14 @@ __global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, cons
     int sv, su;
     tm.super_of(d.k1, sv, su);
     const long long beg = eoff1[d.k1];
-    for (long long r = d.beg + tid; r < d.end; r += 256) {
-        const Rec x = rec[r];
+    for (long long r0 = d.beg + tid; r0 < d.end; r0 += 256 * kUnroll) {
+      Rec xs[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) xs[u] = rec[min(r0 + (long long)u * 256, d.end - 1)];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (r0 + (long long)u * 256 >= d.end) break;
+        const Rec& x = xs[u];
         int fu_a, fu_b, fv_a, fv_b;
         Window s;
         if (!record_tiles<OUT>(x, g, tm, sv, su, fu_a, fu_b, fv_a, fv_b, s)) continue;
@@ -410,6 +439,7 @@ __global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, cons
                 }
             }
         }
+      }
     }
 }
 

@@ -58,6 +58,10 @@ __device__ __forceinline__ Win32 decode32(const Entry32& en, const Geometry& g, 
     return w;
 }
 
+// LDS tiles hold the real parts of the 64x64 cells, then the imaginary parts:
+// an atomic instruction's lanes (random cells) then spread over all banks.
+constexpr int kPlane = kTile * kTile;
+
 // One workgroup per (w, tile_v, tile_u) bucket, one lane per entry.  The
 // Gaussian weight norm*exp(-(du^2+dv^2)/2) (w-stacking_omp.c:13-18,50-52) is
 // evaluated as the separable product norm*exp(-du^2/2)*exp(-dv^2/2): 2*WIN
@@ -108,8 +112,8 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
                 im = x.y;
             }
         }
-        acc[2 * c] = re;
-        acc[2 * c + 1] = im;
+        acc[c] = re;
+        acc[kPlane + c] = im;
     }
     __syncthreads();
 
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
             const Entry64 en = ((const Entry64*)entries)[e];
             const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
             const int a0 = tw.a0, na = tw.na, b0 = tw.b0, nb = tw.nb;
-            double* base = &acc[2 * (b0 * kTile + a0)];
+            double* base = &acc[b0 * kTile + a0];
             double eu[WIN];
 #pragma unroll
             for (int q = 0; q < WIN; q++) {
@@ -132,12 +136,12 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
                     const double dv = (double)(v0 + b0 + kq) + 0.5 - en.pv;
                     const double f = norm * exp(-(dv * dv) * 0.5);
                     const double fr = f * en.vr, fi = f * en.vi;
-                    double* row = base + 2 * kq * kTile;
+                    double* row = base + kq * kTile;
 #pragma unroll
                     for (int q = 0; q < WIN; q++) {
                         if (q <= na) {
-                            atomicAdd(row + 2 * q, eu[q] * fr);
-                            atomicAdd(row + 2 * q + 1, eu[q] * fi);
+                            atomicAdd(row + q, eu[q] * fr);
+                            atomicAdd(row + kPlane + q, eu[q] * fi);
                         }
                     }
                 }
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
             const Entry32 en = ((const Entry32*)entries)[e];
             const Win32 w32 = decode32(en, g, u0, v0);
             const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
-            double* base = &acc[2 * (b0 * kTile + a0)];
+            double* base = &acc[b0 * kTile + a0];
             float eu[WIN];
 #pragma unroll
             for (int q = 0; q < WIN; q++) {
@@ -159,12 +163,12 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
                     const float dv = (float)(w32.iv + kq) + w32.hv;
                     const float f = (float)norm * expf(-(dv * dv) * 0.5f);
                     const float fr = f * en.vr, fi = f * en.vi;
-                    double* row = base + 2 * kq * kTile;
+                    double* row = base + kq * kTile;
 #pragma unroll
                     for (int q = 0; q < WIN; q++) {
                         if (q <= na) {
-                            atomicAdd(row + 2 * q, (double)(eu[q] * fr));
-                            atomicAdd(row + 2 * q + 1, (double)(eu[q] * fi));
+                            atomicAdd(row + q, (double)(eu[q] * fr));
+                            atomicAdd(row + kPlane + q, (double)(eu[q] * fi));
                         }
                     }
                 }
@@ -177,9 +181,9 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
         if (row < trows && col < tcols) {
             const long long o = cell0 + (long long)row * g.nx + col;
             if constexpr (FP64)
-                ((double2*)cube)[o] = double2{acc[2 * c], acc[2 * c + 1]};
+                ((double2*)cube)[o] = double2{acc[c], acc[kPlane + c]};
             else
-                ((float2*)cube)[o] = float2{(float)acc[2 * c], (float)acc[2 * c + 1]};
+                ((float2*)cube)[o] = float2{(float)acc[c], (float)acc[kPlane + c]};
         }
     }
 }
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
             const Entry32 en = entries[e];
             const Win32 w32 = decode32(en, g, u0, v0);
             const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
-            double* base = &dacc[2 * (b0 * kTile + a0)];
+            double* base = &dacc[b0 * kTile + a0];
             float eu[WIN];
 #pragma unroll
             for (int q = 0; q < WIN; q++) {
@@ -278,12 +282,12 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
                     const float dv = (float)(w32.iv + kq) + w32.hv;
                     const float f = (float)norm * expf(-(dv * dv) * 0.5f);
                     const float fr = f * en.vr, fi = f * en.vi;
-                    double* row = base + 2 * kq * kTile;
+                    double* row = base + kq * kTile;
 #pragma unroll
                     for (int q = 0; q < WIN; q++) {
                         if (q <= na) {
-                            atomicAdd(row + 2 * q, (double)(eu[q] * fr));
-                            atomicAdd(row + 2 * q + 1, (double)(eu[q] * fi));
+                            atomicAdd(row + q, (double)(eu[q] * fr));
+                            atomicAdd(row + kPlane + q, (double)(eu[q] * fi));
                         }
                     }
                 }
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
             const int row = c / kTile, col = c % kTile;
             if (row < trows && col < tcols)
                 cube[cell0 + (long long)row * g.nx + col] =
-                    float2{(float)dacc[2 * c], (float)dacc[2 * c + 1]};
+                    float2{(float)dacc[c], (float)dacc[kPlane + c]};
         }
         return;
     }
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
         const Entry32 en = entries[e];
         const Win32 w32 = decode32(en, g, u0, v0);
         const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
-        unsigned long long* base = &acc[2 * (b0 * kTile + a0)];
+        unsigned long long* base = &acc[b0 * kTile + a0];
         double eu[WIN];
 #pragma unroll
         for (int q = 0; q < WIN; q++) {
@@ -322,12 +326,12 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
                 const float dv = (float)(w32.iv + kq) + w32.hv;
                 const float f = (float)norm * expf(-(dv * dv) * 0.5f);
                 const double fr = (double)(f * en.vr) * scale, fi = (double)(f * en.vi) * scale;
-                unsigned long long* row = base + 2 * kq * kTile;
+                unsigned long long* row = base + kq * kTile;
 #pragma unroll
                 for (int q = 0; q < WIN; q++) {
                     if (q <= na) {
-                        atomicAdd(row + 2 * q, to_fixed(eu[q] * fr));
-                        atomicAdd(row + 2 * q + 1, to_fixed(eu[q] * fi));
+                        atomicAdd(row + q, to_fixed(eu[q] * fr));
+                        atomicAdd(row + kPlane + q, to_fixed(eu[q] * fi));
                     }
                 }
             }
@@ -338,8 +342,8 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     for (int c = tid; c < kTile * kTile; c += NT) {
         const int row = c / kTile, col = c % kTile;
         if (row < trows && col < tcols) {
-            const double re = (double)(long long)acc[2 * c] * inv;
-            const double im = (double)(long long)acc[2 * c + 1] * inv;
+            const double re = (double)(long long)acc[c] * inv;
+            const double im = (double)(long long)acc[kPlane + c] * inv;
             cube[cell0 + (long long)row * g.nx + col] = float2{(float)re, (float)im};
         }
     }
