@@ -458,6 +458,12 @@ static void scan(const unsigned* in, long long* out, long long n, long long* par
     hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, st, out, partials, n);
 }
 
+hipError_t fill_u32(unsigned* p, long long n, unsigned v, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t bucket_count(int mode, long long n, const double* uu, const double* vv, const double* ww,
                         const Geometry& g, const Buckets& b, hipStream_t st)
 {

@@ -41,6 +41,7 @@ struct DeviceCtx {
     Buffer meta;                    // counts, offsets, scan partials
     Buffer ents;                    // bucket entries and windows
     Buffer host_io;                 // device copies of drop-in host arrays
+    Buffer occ;                     // per-tile occupancy of the fused imaging path
     long long* status_host = nullptr;  // pinned: [total, err]
     std::map<long long, void*> twiddles;
     std::vector<Prof> prof;
@@ -325,7 +326,7 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
 int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long F, long P,
               const double* uu, const double* vv, const double* ww, const float* vr,
               const float* vi, const float* wt, double dx, double dw, int support, int nx, int ny,
-              void* cube, hipStream_t st)
+              void* cube, hipStream_t st, unsigned long long* occ = nullptr)
 {
     int r = check_grid_args(precision, nw, n, F, P, dx, dw, support, nx, ny);
     if (r) return fail(r);
@@ -351,7 +352,7 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
         return bucket_sort_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, rec, ents, st);
     }));
     HIPCHK(stage(c, kStGrid, st,
-                 [&] { return grid_tiles(precision, g, b, ents, cube, accumulate, st); }));
+                 [&] { return grid_tiles(precision, g, b, ents, cube, accumulate, occ, st); }));
     return LOFAR_OK;
 }
 
@@ -556,9 +557,23 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
     CallGuard cg(stream);
     if (cg.status) return cg.status;
     DeviceCtx& c = *cg.c;
-    int r = grid_impl(c, precision, 0, num_w_planes, num_points, freq_per_chan, polarizations, uu,
-                      vv, ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x,
-                      grid_size_y, cube, cg.st);
+    // FP64: per-tile occupancy bits; the gridder skips empty tiles and the column
+    // pass reads them as zeros (the FP32 column kernel has no registers to spare
+    // for the check at its 1024-thread workgroups, so FP32C writes every tile)
+    int r = LOFAR_OK;
+    unsigned long long* occ = nullptr;
+    if (precision == LOFAR_FP64) {
+        const long long ntv = (grid_size_y + kTile - 1) / kTile;
+        const long long ntu = (grid_size_x + kTile - 1) / kTile;
+        const long long nocc = (long long)num_w_planes * ntu * ((ntv + 63) / 64);
+        r = ensure(c, c.occ, (size_t)nocc * 8 + 256);
+        if (r) return r;
+        occ = (unsigned long long*)c.occ.ptr;
+        HIPCHK(fill_u32((unsigned*)occ, 2 * nocc, 0u, cg.st));
+    }
+    r = grid_impl(c, precision, 0, num_w_planes, num_points, freq_per_chan, polarizations, uu, vv,
+                  ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x, grid_size_y, cube,
+                  cg.st, occ);
     if (r) return r;
     void *twx, *twy;
     r = twiddles(c, grid_size_x, precision, &twx);
@@ -571,7 +586,7 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
     if (precision == LOFAR_FP64) {
         HIPCHK(stage(c, kStCols, cg.st, [&] {
             return fft_cols<double>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                    1.0, cg.st);
+                                    1.0, cg.st, occ);
         }));
         HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
             return rows_phase<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,

@@ -48,12 +48,13 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_fft_rows(const C2<Real>* 
 // W*8 bytes contiguous per row access), LDS laid out [slot][W].  Workgroups
 // are numbered so that the strips one XCD runs at a time are adjacent in
 // memory and share its L2 lines.
-template <typename Real, int N, int E, int W, int ROUNDS, int SIGN>
+template <typename Real, int N, int E, int W, int ROUNDS, int SIGN, bool OCC>
 __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Real>* in,
                                                                  C2<Real>* out,
                                                                  const C2<Real>* tw, int nx,
                                                                  int strips_per_plane,
-                                                                 long long nstrips, Real scale)
+                                                                 long long nstrips, Real scale,
+                                                                 const unsigned long long* occ)
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
@@ -67,9 +68,26 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
     const bool active = u0 + c < nx;  // the last strip of a plane may be partial
     const long long base = plane * (long long)N * nx + (active ? u0 + c : 0);
     T2 v[P::E];
+    if constexpr (OCC) {
+        // occupancy bits of this strip's tile column (bit tv of word tv/64): rows of
+        // tiles the gridder left unwritten because they hold no samples read as zeros
+        constexpr int NWD = (N / kTile + 63) / 64;
+        const int ntu = (nx + kTile - 1) / kTile;
+        const unsigned long long* om = occ + (plane * ntu + u0 / kTile) * (NWD > 0 ? NWD : 1);
+        const unsigned long long m0 = om[0];
+        const unsigned long long m1 = NWD > 1 ? om[1] : 0ull;
 #pragma unroll
-    for (int q = 0; q < P::E; q++)
-        v[q] = active ? in[base + (long long)P::in_index(t, q) * nx] : T2{0, 0};
+        for (int q = 0; q < P::E; q++) {
+            const int row = P::in_index(t, q);
+            const int tv = row / kTile;
+            const bool live = active && (((tv < 64 ? m0 : m1) >> (tv & 63)) & 1ull);
+            v[q] = live ? in[base + (long long)row * nx] : T2{0, 0};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < P::E; q++)
+            v[q] = active ? in[base + (long long)P::in_index(t, q) * nx] : T2{0, 0};
+    }
     stages<T2, N, E, SIGN, 1, ROUNDS>(v, lds + c % WR, WR, t, tw, c / WR);
     if (active) {
 #pragma unroll
@@ -310,7 +328,8 @@ static hipError_t launch_rows_n(int sign, const void* in, void* out, const void*
 
 template <typename Real, int N, int E, int WREQ, int RREQ>
 static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const void* tw, int nx,
-                                  int nw, double scale, hipStream_t st)
+                                  int nw, double scale, const unsigned long long* occ,
+                                  hipStream_t st)
 {
     using T2 = C2<Real>;
     constexpr int TPL = Plan<N, E>::TPL;
@@ -324,9 +343,14 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
     constexpr int ROUNDS = R1 < W ? R1 : W;
     const int strips = (nx + W - 1) / W;
     const long long nstrips = (long long)strips * nw;
-    auto k = sign > 0 ? k_fft_cols<Real, N, E, W, ROUNDS, 1> : k_fft_cols<Real, N, E, W, ROUNDS, -1>;
+    // the occupancy-checking variant exists for FP64 only (FP32 strips would spill)
+    constexpr bool kOcc = sizeof(Real) == 8;
+    if (occ && !kOcc) return hipErrorInvalidValue;
+    auto k = sign > 0 ? (occ ? k_fft_cols<Real, N, E, W, ROUNDS, 1, kOcc>
+                             : k_fft_cols<Real, N, E, W, ROUNDS, 1, false>)
+                      : k_fft_cols<Real, N, E, W, ROUNDS, -1, false>;
     hipLaunchKernelGGL(k, dim3((unsigned)nstrips), dim3(W * TPL), 0, st, (const T2*)in, (T2*)out,
-                       (const T2*)tw, nx, strips, nstrips, (Real)scale);
+                       (const T2*)tw, nx, strips, nstrips, (Real)scale, occ);
     return hipGetLastError();
 }
 
@@ -334,16 +358,17 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
 // LDS fits ~128 KiB in one round; W64: 64-byte row segments.
 template <typename Real, int N>
 static hipError_t launch_cols_n(int sign, const void* in, void* out, const void* tw, int nx,
-                                int nw, double scale, hipStream_t st)
+                                int nw, double scale, const unsigned long long* occ,
+                                hipStream_t st)
 {
     constexpr int B = Plan<N, 16>::LDS * 2 * (int)sizeof(Real);
     constexpr int W2 = 140000 / B >= 1 ? (140000 / B > 16 ? 16 : 140000 / B) : 1;
     constexpr int W64 = 64 / (2 * (int)sizeof(Real));
     switch (col_cfg()) {
-        case 1: return launch_cols_cfg<Real, N, 16, W2, 1>(sign, in, out, tw, nx, nw, scale, st);
-        case 2: return launch_cols_cfg<Real, N, 32, W64, 4>(sign, in, out, tw, nx, nw, scale, st);
-        case 3: return launch_cols_cfg<Real, N, 16, W64, 2>(sign, in, out, tw, nx, nw, scale, st);
-        default: return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, st);
+        case 1: return launch_cols_cfg<Real, N, 16, W2, 1>(sign, in, out, tw, nx, nw, scale, occ, st);
+        case 2: return launch_cols_cfg<Real, N, 32, W64, 4>(sign, in, out, tw, nx, nw, scale, occ, st);
+        case 3: return launch_cols_cfg<Real, N, 16, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, st);
+        default: return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, st);
     }
 }
 
@@ -428,9 +453,9 @@ hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx,
 
 template <typename Real>
 hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx, int ny, int nw,
-                    double scale, hipStream_t st)
+                    double scale, hipStream_t st, const unsigned long long* occ)
 {
-#define CALL_(n) launch_cols_n<Real, n>(sign, in, out, tw, nx, nw, scale, st)
+#define CALL_(n) launch_cols_n<Real, n>(sign, in, out, tw, nx, nw, scale, occ, st)
     LOFAR_FFT_SWITCH(ny, CALL_)
 #undef CALL_
 }
@@ -473,9 +498,9 @@ template hipError_t fft_rows<double>(int, const void*, void*, const void*, int, 
 template hipError_t fft_rows<float>(int, const void*, void*, const void*, int, long long, double,
                                     hipStream_t);
 template hipError_t fft_cols<double>(int, const void*, void*, const void*, int, int, int, double,
-                                     hipStream_t);
+                                     hipStream_t, const unsigned long long*);
 template hipError_t fft_cols<float>(int, const void*, void*, const void*, int, int, int, double,
-                                    hipStream_t);
+                                    hipStream_t, const unsigned long long*);
 template hipError_t rows_phase<double>(const void*, const void*, int, int, int, const PhaseParams&,
                                        double, double*, double*, int, hipStream_t);
 template hipError_t rows_phase<float>(const void*, const void*, int, int, int, const PhaseParams&,

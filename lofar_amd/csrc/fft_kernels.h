@@ -378,8 +378,10 @@ template <typename Real>
 hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx, long long lines,
                     double scale, hipStream_t st);
 template <typename Real>
+// occ (optional, forward transforms only): per-tile occupancy bits from
+// grid_tiles; rows of empty tiles are read as zeros
 hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx, int ny, int nw,
-                    double scale, hipStream_t st);
+                    double scale, hipStream_t st, const unsigned long long* occ = nullptr);
 template <typename Real>
 hipError_t rows_phase(const void* cube, const void* tw, int nx, int ny, int nw,
                       const PhaseParams& pp, double scale, double* re, double* im, int accumulate,

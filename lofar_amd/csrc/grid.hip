@@ -69,7 +69,7 @@ constexpr int kPlane = kTile * kTile;
 template <int WIN, bool FP64, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const long long* offsets,
                                                     const void* entries, void* cube,
-                                                    int accumulate)
+                                                    int accumulate, unsigned long long* occ)
 {
     __shared__ double acc[2 * kTile * kTile];
     const long long bkt = blockIdx.x;
@@ -82,6 +82,14 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
     const long long cell0 = (w * g.ny + v0) * (long long)g.nx + u0;
     const int tid = threadIdx.x;
 
+    if (occ) {
+        // fused imaging: set the tile's occupancy bit (word (w, tu, tv/64), bit
+        // tv%64, cleared beforehand); the column transform reads an empty tile as
+        // zeros, so its cells are left unwritten
+        if (beg == end) return;
+        if (tid == 0)
+            atomicOr(&occ[(w * g.ntu + tu) * ((g.ntv + 63) / 64) + tv / 64], 1ull << (tv % 64));
+    }
     if (beg == end) {
         if (!accumulate) {
             for (int c = tid; c < kTile * kTile; c += NT) {
@@ -218,7 +226,8 @@ __device__ __forceinline__ unsigned long long to_fixed(double y)
 
 template <int WIN, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, const long long* offsets,
-                                                       const Entry32* entries, float2* cube)
+                                                       const Entry32* entries, float2* cube,
+                                                       unsigned long long* occ)
 {
     __shared__ unsigned long long acc[2 * kTile * kTile];
     __shared__ float wred[NT / 64];
@@ -233,6 +242,14 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     const int tid = threadIdx.x;
     const double norm = 0.5 / LOFAR_REF_PI;
 
+    if (occ) {
+        // fused imaging: set the tile's occupancy bit (word (w, tu, tv/64), bit
+        // tv%64, cleared beforehand); the column transform reads an empty tile as
+        // zeros, so its cells are left unwritten
+        if (beg == end) return;
+        if (tid == 0)
+            atomicOr(&occ[(w * g.ntu + tu) * ((g.ntv + 63) / 64) + tv / 64], 1ull << (tv % 64));
+    }
     if (beg == end) {
         for (int c = tid; c < kTile * kTile; c += NT) {
             const int row = c / kTile, col = c % kTile;
@@ -469,41 +486,41 @@ static int grid_nt()
 
 template <int WIN, int NT>
 static void launch_grid_nt(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                           void* cube, int accumulate, hipStream_t st)
+                           void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
 {
     const dim3 grid((unsigned)b.map.nbuckets());
     if (precision == LOFAR_FP64)
         hipLaunchKernelGGL((k_grid_tiles<WIN, true, NT>), grid, dim3(NT), 0, st, g, b.map,
-                           b.offsets, entries, cube, accumulate);
+                           b.offsets, entries, cube, accumulate, occ);
     else if (!accumulate && grid_cfg() == 0)
         hipLaunchKernelGGL((k_grid_tiles_fx<WIN, NT>), grid, dim3(NT), 0, st, g, b.map, b.offsets,
-                           (const Entry32*)entries, (float2*)cube);
+                           (const Entry32*)entries, (float2*)cube, occ);
     else
         hipLaunchKernelGGL((k_grid_tiles<WIN, false, NT>), grid, dim3(NT), 0, st, g, b.map,
-                           b.offsets, entries, cube, accumulate);
+                           b.offsets, entries, cube, accumulate, occ);
 }
 
 template <int WIN>
 static void launch_grid(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                        void* cube, int accumulate, hipStream_t st)
+                        void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
 {
     switch (grid_nt()) {
-        case 1024: return launch_grid_nt<WIN, 1024>(precision, g, b, entries, cube, accumulate, st);
-        case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, st);
-        default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, st);
+        case 1024: return launch_grid_nt<WIN, 1024>(precision, g, b, entries, cube, accumulate, occ, st);
+        case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, occ, st);
+        default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, occ, st);
     }
 }
 
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                      void* cube, int accumulate, hipStream_t st)
+                      void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
 {
     const int win = 2 * g.K + 1;
     if (win <= 7)
-        launch_grid<7>(precision, g, b, entries, cube, accumulate, st);
+        launch_grid<7>(precision, g, b, entries, cube, accumulate, occ, st);
     else if (win <= 9)
-        launch_grid<9>(precision, g, b, entries, cube, accumulate, st);
+        launch_grid<9>(precision, g, b, entries, cube, accumulate, occ, st);
     else if (win <= 17)
-        launch_grid<17>(precision, g, b, entries, cube, accumulate, st);
+        launch_grid<17>(precision, g, b, entries, cube, accumulate, occ, st);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -137,8 +137,13 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
                             const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
                             hipStream_t st);
+// occ (optional, nw*ntu*ceil(ntv/64) words, zeroed by the caller): bit tv%64 of
+// word (w, tu, tv/64) is set for every tile holding samples; empty tiles are then
+// left unwritten (for a consumer that reads them as zeros).
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                      void* cube, int accumulate, hipStream_t st);
+                      void* cube, int accumulate, unsigned long long* occ, hipStream_t st);
+// fills n words with v (stream-ordered, no memset node)
+hipError_t fill_u32(unsigned* p, long long n, unsigned v, hipStream_t st);
 hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, const DEntry* entries,
                         const void* cube, const float* weight, long long F, long long P,
                         void* model_re, void* model_im, hipStream_t st);

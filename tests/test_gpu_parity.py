@@ -334,6 +334,24 @@ def test_image_pipeline_matches_oracle(cuda, n, nx, nw, support):
     assert e32 < 1e-6
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-6)])
+def test_image_ignores_stale_scratch_cube(cuda, precision, tol):
+    """The fused path may leave tiles without samples unwritten (their rows are read as
+    zeros by the column transform): stale NaNs in the caller's scratch cube must not leak
+    into the image.  Sparse coverage leaves most tiles of most planes empty."""
+    import torch
+    from lofar_amd import imager
+    d = synth(3000, 512, 8, seed=77)
+    want = O.image(8, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
+                   512, 512, d["resolution"], d["wmin"], d["wmax"])
+    cube = torch.full((8, 512, 512), float("nan"), dtype=imager.cube_dtype(precision),
+                      device=cuda)
+    for _ in range(2):
+        r, i = imager.image(*dev_inputs(d, cuda), 8, 7, 512, 512, d["resolution"], d["wmin"],
+                            d["wmax"], precision=precision, cube=cube)
+        assert rel(H(r) + 1j * H(i), want) < tol
+
+
 def test_dropin_chain_matches_oracle(cuda):
     # wstack -> fftw_data -> phase_correction through the host drop-ins, as gridding.c,
     # fourier_transform.c:7-74 and :173 call them
