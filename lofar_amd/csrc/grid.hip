@@ -366,6 +366,158 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     }
 }
 
+// FP64 gridding with exact fixed-point accumulation: as k_grid_tiles_fx, with
+// double visibilities and a 62-bit range (S = 62 - hb - ex); every contribution
+// eu*fr is the double product of the reference's factors, rounded to the
+// nearest multiple of 2^-S, and the integer sums are exact and independent of
+// order.  The quantum is ~2^-(60-log2(entries)) of the tile's largest
+// contribution and the rounding is unbiased, so the cells agree with a
+// double-precision sum to ~1e-15 of the tile's largest contribution.
+constexpr int kFixedBits64 = 62;
+
+// round(y) as a two's-complement 64-bit value, |y| < 2^62: floor(y + 1/2) split
+// into a high and a low 32-bit part, each exact
+__device__ __forceinline__ unsigned long long to_fixed62(double y)
+{
+    const double t = y + 0.5;
+    const double hi = floor(t * 0x1p-32);
+    const double lo = fma(-hi, 0x1p32, t);
+    return ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)(unsigned)lo;
+}
+
+template <int WIN, int NT>
+__global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
+                                                         const long long* offsets,
+                                                         const Entry64* entries, double2* cube,
+                                                         unsigned long long* occ)
+{
+    __shared__ unsigned long long acc[2 * kPlane];
+    __shared__ double wred[NT / 64];
+    __shared__ int all_finite;
+    const long long bkt = blockIdx.x;
+    long long w;
+    int tv, tu;
+    if (!tm.decode(bkt, g, w, tv, tu)) return;
+    const int u0 = tu * kTile, v0 = tv * kTile;
+    const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
+    const long long beg = offsets[bkt], end = offsets[bkt + 1];
+    const long long cell0 = (w * g.ny + v0) * (long long)g.nx + u0;
+    const int tid = threadIdx.x;
+    const double norm = 0.5 / LOFAR_REF_PI;
+
+    if (occ) {
+        if (beg == end) return;
+        if (tid == 0)
+            atomicOr(&occ[(w * g.ntu + tu) * ((g.ntv + 63) / 64) + tv / 64], 1ull << (tv % 64));
+    }
+    if (beg == end) {
+        for (int c = tid; c < kPlane; c += NT) {
+            const int row = c / kTile, col = c % kTile;
+            if (row < trows && col < tcols)
+                cube[cell0 + (long long)row * g.nx + col] = double2{0.0, 0.0};
+        }
+        return;
+    }
+    double m = 0.0;
+    bool finite = true;
+    for (long long e = beg + tid; e < end; e += NT) {
+        const Entry64 en = entries[e];
+        finite = finite && isfinite(en.vr) && isfinite(en.vi);
+        m = fmax(m, fmax(fabs(en.vr), fabs(en.vi)));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) wred[wave] = m;
+    if (tid == 0) all_finite = 1;
+    __syncthreads();
+    if (!finite) all_finite = 0;
+    m = wred[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; i++) m = fmax(m, wred[i]);
+    for (int c = tid; c < 2 * kPlane; c += NT) acc[c] = 0ull;  // also 0.0 as double
+    __syncthreads();
+    if (!all_finite) {
+        // floating-point accumulation (ds_add_f64), as k_grid_tiles<WIN, true>
+        double* dacc = (double*)acc;
+        for (long long e = beg + tid; e < end; e += NT) {
+            const Entry64 en = entries[e];
+            const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
+            double* base = &dacc[tw.b0 * kTile + tw.a0];
+            double eu[WIN];
+#pragma unroll
+            for (int q = 0; q < WIN; q++) {
+                const double du = (double)(u0 + tw.a0 + q) + 0.5 - en.pu;
+                eu[q] = (q <= tw.na) ? exp(-(du * du) * 0.5) : 0.0;
+            }
+#pragma unroll
+            for (int kq = 0; kq < WIN; kq++) {
+                if (kq <= tw.nb) {
+                    const double dv = (double)(v0 + tw.b0 + kq) + 0.5 - en.pv;
+                    const double f = norm * exp(-(dv * dv) * 0.5);
+                    const double fr = f * en.vr, fi = f * en.vi;
+                    double* row = base + kq * kTile;
+#pragma unroll
+                    for (int q = 0; q < WIN; q++) {
+                        if (q <= tw.na) {
+                            atomicAdd(row + q, eu[q] * fr);
+                            atomicAdd(row + kPlane + q, eu[q] * fi);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int c = tid; c < kPlane; c += NT) {
+            const int row = c / kTile, col = c % kTile;
+            if (row < trows && col < tcols)
+                cube[cell0 + (long long)row * g.nx + col] = double2{dacc[c], dacc[kPlane + c]};
+        }
+        return;
+    }
+    int ex = 0;
+    frexp(norm * (m > 0.0 ? m : 1.0), &ex);  // norm*m < 2^ex
+    int hb = 2;
+    for (long long cnt = end - beg; cnt > 1; cnt >>= 1) hb++;
+    const int S = kFixedBits64 - hb - ex;
+    const double scale = ldexp(1.0, S);
+
+    for (long long e = beg + tid; e < end; e += NT) {
+        const Entry64 en = entries[e];
+        const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
+        unsigned long long* base = &acc[tw.b0 * kTile + tw.a0];
+        double eu[WIN];
+#pragma unroll
+        for (int q = 0; q < WIN; q++) {
+            const double du = (double)(u0 + tw.a0 + q) + 0.5 - en.pu;
+            eu[q] = (q <= tw.na) ? exp(-(du * du) * 0.5) : 0.0;
+        }
+#pragma unroll
+        for (int kq = 0; kq < WIN; kq++) {
+            if (kq <= tw.nb) {
+                const double dv = (double)(v0 + tw.b0 + kq) + 0.5 - en.pv;
+                const double f = norm * exp(-(dv * dv) * 0.5);
+                const double fr = f * en.vr * scale, fi = f * en.vi * scale;
+                unsigned long long* row = base + kq * kTile;
+#pragma unroll
+                for (int q = 0; q < WIN; q++) {
+                    if (q <= tw.na) {
+                        atomicAdd(row + q, to_fixed62(eu[q] * fr));
+                        atomicAdd(row + kPlane + q, to_fixed62(eu[q] * fi));
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const double inv = ldexp(1.0, -S);
+    for (int c = tid; c < kPlane; c += NT) {
+        const int row = c / kTile, col = c % kTile;
+        if (row < trows && col < tcols)
+            cube[cell0 + (long long)row * g.nx + col] =
+                double2{(double)(long long)acc[c] * inv, (double)(long long)acc[kPlane + c] * inv};
+    }
+}
+
 // ---------------------------------------------------------------------------
 // degridding (adjoint of gridding)
 
@@ -460,8 +612,8 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
 // ---------------------------------------------------------------------------
 // host dispatch
 
-// FP32C accumulation: 0 = exact 64-bit fixed point (default), 1 = ds_add_f64
-// (LOFAR_GRID_CFG overrides, for measurement)
+// Accumulation without accumulate: 0 = exact 64-bit fixed point (default),
+// 1 = ds_add_f64 (LOFAR_GRID_CFG overrides, for measurement)
 static int grid_cfg()
 {
     static int cfg = -1;
@@ -489,7 +641,10 @@ static void launch_grid_nt(int precision, const Geometry& g, const Buckets& b, c
                            void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
 {
     const dim3 grid((unsigned)b.map.nbuckets());
-    if (precision == LOFAR_FP64)
+    if (precision == LOFAR_FP64 && !accumulate && grid_cfg() == 0)
+        hipLaunchKernelGGL((k_grid_tiles_fx64<WIN, NT>), grid, dim3(NT), 0, st, g, b.map,
+                           b.offsets, (const Entry64*)entries, (double2*)cube, occ);
+    else if (precision == LOFAR_FP64)
         hipLaunchKernelGGL((k_grid_tiles<WIN, true, NT>), grid, dim3(NT), 0, st, g, b.map,
                            b.offsets, entries, cube, accumulate, occ);
     else if (!accumulate && grid_cfg() == 0)
