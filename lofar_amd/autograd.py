@@ -42,10 +42,11 @@ class ImagingOperator:
     """A = W G^T F^H P^H for fixed samples and geometry, with one reused device cube.
 
     uu, vv, ww: float64 CUDA tensors [N] normalised as the reference's inputs;
-    weight: float32 CUDA tensor [N] or None (unit weights)."""
+    weight: float32 CUDA tensor [N] or None (unit weights); cube: optional scratch
+    [nw, ny, nx] of the precision's complex dtype."""
 
     def __init__(self, uu, vv, ww, weight, num_w_planes, w_support, grid_size_x, grid_size_y,
-                 resolution, wmin, wmax, precision="fp64"):
+                 resolution, wmin, wmax, precision="fp64", cube=None):
         torch = _torch()
         self.uu, self.vv, self.ww = uu, vv, ww
         n = uu.numel()
@@ -56,8 +57,10 @@ class ImagingOperator:
         self.nx, self.ny = grid_size_x, grid_size_y
         self.resolution, self.wmin, self.wmax = resolution, wmin, wmax
         self.precision = imager._prec(precision)
-        self.cube = torch.empty((num_w_planes, grid_size_y, grid_size_x),
-                                dtype=imager.cube_dtype(self.precision), device=uu.device)
+        if cube is None:
+            cube = torch.empty((num_w_planes, grid_size_y, grid_size_x),
+                               dtype=imager.cube_dtype(self.precision), device=uu.device)
+        self.cube = cube  # scratch shared by forward and adjoint
 
     @property
     def num_samples(self):
