@@ -326,7 +326,7 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
 int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long F, long P,
               const double* uu, const double* vv, const double* ww, const float* vr,
               const float* vi, const float* wt, double dx, double dw, int support, int nx, int ny,
-              void* cube, hipStream_t st, unsigned long long* occ = nullptr)
+              void* cube, hipStream_t st, unsigned long long* occ = nullptr, int sm_log = -1)
 {
     int r = check_grid_args(precision, nw, n, F, P, dx, dw, support, nx, ny);
     if (r) return fail(r);
@@ -352,7 +352,7 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
         return bucket_sort_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, rec, ents, st);
     }));
     HIPCHK(stage(c, kStGrid, st,
-                 [&] { return grid_tiles(precision, g, b, ents, cube, accumulate, occ, st); }));
+                 [&] { return grid_tiles(precision, g, b, ents, cube, accumulate, occ, sm_log, st); }));
     return LOFAR_OK;
 }
 
@@ -427,6 +427,19 @@ void report_once(const char* what, int code)
                 lofar_error_string(code));
         printed = true;
     }
+}
+
+// strip width (log2 columns) of the fused path's scratch cube; -1 = row-major
+int sm_log_cfg(int precision)
+{
+    static int v = -2;
+    if (v == -2) {
+        const char* e = getenv("LOFAR_SM_LOG");
+        v = e ? atoi(e) : -3;
+        if (v < -1 || v > 6) v = -3;
+    }
+    if (v != -3) return v;
+    return precision == LOFAR_FP64 ? 6 : -1;
 }
 
 }  // namespace
@@ -571,9 +584,20 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
         occ = (unsigned long long*)c.occ.ptr;
         HIPCHK(fill_u32((unsigned*)occ, 2 * nocc, 0u, cg.st));
     }
+    // FP64: the scratch cube is laid out strip-major, in strips of 2^sm_log whole
+    // columns (the 64-column tile width by default), so every tile flush is one
+    // contiguous block and the column transforms read 64-byte runs 1 KiB apart instead
+    // of a plane row apart; FP32C keeps row-major planes (no gain measured there).
+    // LOFAR_SM_LOG overrides, for measurement (-1: row-major; at most the tile width).
+    const int sm_pref = sm_log_cfg(precision);
+    const int sm_log = (sm_pref >= 0 && (1 << sm_pref) <= kTile &&
+                        grid_size_x % (1 << sm_pref) == 0 &&
+                        (1 << sm_pref) % (precision == LOFAR_FP64 ? 4 : 8) == 0)
+                           ? sm_pref
+                           : -1;
     r = grid_impl(c, precision, 0, num_w_planes, num_points, freq_per_chan, polarizations, uu, vv,
                   ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x, grid_size_y, cube,
-                  cg.st, occ);
+                  cg.st, occ, sm_log);
     if (r) return r;
     void *twx, *twy;
     r = twiddles(c, grid_size_x, precision, &twx);
@@ -586,20 +610,20 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
     if (precision == LOFAR_FP64) {
         HIPCHK(stage(c, kStCols, cg.st, [&] {
             return fft_cols<double>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                    1.0, cg.st, occ);
+                                    1.0, cg.st, occ, sm_log);
         }));
         HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
             return rows_phase<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
-                                      scale, image_real, image_imag, accumulate, cg.st);
+                                      scale, image_real, image_imag, accumulate, cg.st, sm_log);
         }));
     } else {
         HIPCHK(stage(c, kStCols, cg.st, [&] {
             return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                   1.0, cg.st);
+                                   1.0, cg.st, nullptr, sm_log);
         }));
         HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
             return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp, scale,
-                                     image_real, image_imag, accumulate, cg.st);
+                                     image_real, image_imag, accumulate, cg.st, sm_log);
         }));
     }
     return LOFAR_OK;

@@ -54,7 +54,8 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
                                                                  const C2<Real>* tw, int nx,
                                                                  int strips_per_plane,
                                                                  long long nstrips, Real scale,
-                                                                 const unsigned long long* occ)
+                                                                 const unsigned long long* occ,
+                                                                 int sm_log)
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
@@ -66,7 +67,13 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
     const int c = threadIdx.x % W;
     const int t = threadIdx.x / W;
     const bool active = u0 + c < nx;  // the last strip of a plane may be partial
-    const long long base = plane * (long long)N * nx + (active ? u0 + c : 0);
+    // row-major planes: rows nx apart; strip-major planes ([nx/L][N][L], L = 2^sm_log,
+    // a multiple of W): rows L apart inside the strip holding these columns
+    const long long rs = sm_log >= 0 ? (1LL << sm_log) : nx;
+    const long long base =
+        plane * (long long)N * nx +
+        (sm_log >= 0 ? ((long long)(u0 >> sm_log) * N << sm_log) + (u0 & ((1 << sm_log) - 1)) + c
+                     : (active ? u0 + c : 0));
     T2 v[P::E];
     if constexpr (OCC) {
         // occupancy bits of this strip's tile column (bit tv of word tv/64): rows of
@@ -81,12 +88,12 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
             const int row = P::in_index(t, q);
             const int tv = row / kTile;
             const bool live = active && (((tv < 64 ? m0 : m1) >> (tv & 63)) & 1ull);
-            v[q] = live ? in[base + (long long)row * nx] : T2{0, 0};
+            v[q] = live ? in[base + (long long)row * rs] : T2{0, 0};
         }
     } else {
 #pragma unroll
         for (int q = 0; q < P::E; q++)
-            v[q] = active ? in[base + (long long)P::in_index(t, q) * nx] : T2{0, 0};
+            v[q] = active ? in[base + (long long)P::in_index(t, q) * rs] : T2{0, 0};
     }
     stages<T2, N, E, SIGN, 1, ROUNDS>(v, lds + c % WR, WR, t, tw, c / WR);
     if (active) {
@@ -95,7 +102,7 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
             T2 x = v[e];
             x.x *= scale;
             x.y *= scale;
-            out[base + (long long)P::out_index(t, e) * nx] = x;
+            out[base + (long long)P::out_index(t, e) * rs] = x;
         }
     }
 }
@@ -186,7 +193,7 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
                                                                 const C2<Real>* tw, int ny, int nw,
                                                                 PhaseParams pp, double scale,
                                                                 double* img_re, double* img_im,
-                                                                int accumulate)
+                                                                int accumulate, int sm_log)
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
@@ -206,12 +213,21 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
     T2 twr[NTW > 0 ? NTW : 1];
     if constexpr (TWREG && NTW > 0) load_twiddles<T2, N, E, 1>(twr, t, tw);
     const long long plane_stride = (long long)ny * N;
-    const T2* src = cube + img_row;
+    // element u of the row: at u (row-major planes) or, in strip-major planes
+    // ([N/L][ny][L], L = 2^sm_log), at (u/L)*ny*L + iv*L + u%L
+    long long off[P::E];
+#pragma unroll
+    for (int q = 0; q < P::E; q++) {
+        const int u = P::in_index(t, q);
+        off[q] = sm_log < 0 ? img_row + u
+                            : ((((long long)(u >> sm_log) * ny + iv) << sm_log) |
+                               (u & ((1 << sm_log) - 1)));
+    }
     T2 v[P::E];
     for (int iw = 0; iw < nw; iw++) {
-        const T2* psrc = src + (long long)iw * plane_stride;
+        const T2* psrc = cube + (long long)iw * plane_stride;
 #pragma unroll
-        for (int q = 0; q < P::E; q++) v[q] = psrc[P::in_index(t, q)];
+        for (int q = 0; q < P::E; q++) v[q] = psrc[off[q]];
         if constexpr (TWREG)
             stages_rt<T2, N, E, 1, 1>(v, lds, t, twr);
         else
@@ -328,7 +344,7 @@ static hipError_t launch_rows_n(int sign, const void* in, void* out, const void*
 
 template <typename Real, int N, int E, int WREQ, int RREQ>
 static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const void* tw, int nx,
-                                  int nw, double scale, const unsigned long long* occ,
+                                  int nw, double scale, const unsigned long long* occ, int sm_log,
                                   hipStream_t st)
 {
     using T2 = C2<Real>;
@@ -349,8 +365,11 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
     auto k = sign > 0 ? (occ ? k_fft_cols<Real, N, E, W, ROUNDS, 1, kOcc>
                              : k_fft_cols<Real, N, E, W, ROUNDS, 1, false>)
                       : k_fft_cols<Real, N, E, W, ROUNDS, -1, false>;
+    // strip-major planes: whole strips of a power-of-two width that W divides
+    if (sm_log >= 0 && ((1 << sm_log) % W != 0 || nx % (1 << sm_log) != 0))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, dim3((unsigned)nstrips), dim3(W * TPL), 0, st, (const T2*)in, (T2*)out,
-                       (const T2*)tw, nx, strips, nstrips, (Real)scale, occ);
+                       (const T2*)tw, nx, strips, nstrips, (Real)scale, occ, sm_log);
     return hipGetLastError();
 }
 
@@ -358,17 +377,19 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
 // LDS fits ~128 KiB in one round; W64: 64-byte row segments.
 template <typename Real, int N>
 static hipError_t launch_cols_n(int sign, const void* in, void* out, const void* tw, int nx,
-                                int nw, double scale, const unsigned long long* occ,
+                                int nw, double scale, const unsigned long long* occ, int sm_log,
                                 hipStream_t st)
 {
     constexpr int B = Plan<N, 16>::LDS * 2 * (int)sizeof(Real);
     constexpr int W2 = 140000 / B >= 1 ? (140000 / B > 16 ? 16 : 140000 / B) : 1;
     constexpr int W64 = 64 / (2 * (int)sizeof(Real));
+    if (sm_log >= 0)
+        return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
     switch (col_cfg()) {
-        case 1: return launch_cols_cfg<Real, N, 16, W2, 1>(sign, in, out, tw, nx, nw, scale, occ, st);
-        case 2: return launch_cols_cfg<Real, N, 32, W64, 4>(sign, in, out, tw, nx, nw, scale, occ, st);
-        case 3: return launch_cols_cfg<Real, N, 16, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, st);
-        default: return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, st);
+        case 1: return launch_cols_cfg<Real, N, 16, W2, 1>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
+        case 2: return launch_cols_cfg<Real, N, 32, W64, 4>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
+        case 3: return launch_cols_cfg<Real, N, 16, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
+        default: return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
     }
 }
 
@@ -385,12 +406,12 @@ static int phase_cfg()
 template <typename Real, int N, int E, bool RECUR, bool TWREG>
 static hipError_t launch_rows_phase_cfg(const void* cube, const void* tw, int ny, int nw,
                                         const PhaseParams& pp, double scale, double* re,
-                                        double* im, int accumulate, hipStream_t st)
+                                        double* im, int accumulate, int sm_log, hipStream_t st)
 {
     using T2 = C2<Real>;
     hipLaunchKernelGGL((k_rows_phase<Real, N, E, RECUR, TWREG>), dim3(ny), dim3(Plan<N, E>::TPL),
                        0, st, (const T2*)cube, (const T2*)tw, ny, nw, pp, scale, re, im,
-                       accumulate);
+                       accumulate, sm_log);
     return hipGetLastError();
 }
 
@@ -399,9 +420,9 @@ static hipError_t launch_rows_phase_cfg(const void* cube, const void* tw, int ny
 template <typename Real, int N>
 static hipError_t launch_rows_phase_n(const void* cube, const void* tw, int ny, int nw,
                                       const PhaseParams& pp, double scale, double* re, double* im,
-                                      int accumulate, hipStream_t st)
+                                      int accumulate, int sm_log, hipStream_t st)
 {
-#define ARGS_ cube, tw, ny, nw, pp, scale, re, im, accumulate, st
+#define ARGS_ cube, tw, ny, nw, pp, scale, re, im, accumulate, sm_log, st
     switch (phase_cfg()) {
         case 1: return launch_rows_phase_cfg<Real, N, 8, false, true>(ARGS_);
         case 2: return launch_rows_phase_cfg<Real, N, 8, true, false>(ARGS_);
@@ -453,9 +474,9 @@ hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx,
 
 template <typename Real>
 hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx, int ny, int nw,
-                    double scale, hipStream_t st, const unsigned long long* occ)
+                    double scale, hipStream_t st, const unsigned long long* occ, int sm_log)
 {
-#define CALL_(n) launch_cols_n<Real, n>(sign, in, out, tw, nx, nw, scale, occ, st)
+#define CALL_(n) launch_cols_n<Real, n>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st)
     LOFAR_FFT_SWITCH(ny, CALL_)
 #undef CALL_
 }
@@ -463,9 +484,9 @@ hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx,
 template <typename Real>
 hipError_t rows_phase(const void* cube, const void* tw, int nx, int ny, int nw,
                       const PhaseParams& pp, double scale, double* re, double* im, int accumulate,
-                      hipStream_t st)
+                      hipStream_t st, int sm_log)
 {
-#define CALL_(n) launch_rows_phase_n<Real, n>(cube, tw, ny, nw, pp, scale, re, im, accumulate, st)
+#define CALL_(n) launch_rows_phase_n<Real, n>(cube, tw, ny, nw, pp, scale, re, im, accumulate, sm_log, st)
     LOFAR_FFT_SWITCH(nx, CALL_)
 #undef CALL_
 }
@@ -498,13 +519,13 @@ template hipError_t fft_rows<double>(int, const void*, void*, const void*, int, 
 template hipError_t fft_rows<float>(int, const void*, void*, const void*, int, long long, double,
                                     hipStream_t);
 template hipError_t fft_cols<double>(int, const void*, void*, const void*, int, int, int, double,
-                                     hipStream_t, const unsigned long long*);
+                                     hipStream_t, const unsigned long long*, int);
 template hipError_t fft_cols<float>(int, const void*, void*, const void*, int, int, int, double,
-                                    hipStream_t, const unsigned long long*);
+                                    hipStream_t, const unsigned long long*, int);
 template hipError_t rows_phase<double>(const void*, const void*, int, int, int, const PhaseParams&,
-                                       double, double*, double*, int, hipStream_t);
+                                       double, double*, double*, int, hipStream_t, int);
 template hipError_t rows_phase<float>(const void*, const void*, int, int, int, const PhaseParams&,
-                                      double, double*, double*, int, hipStream_t);
+                                      double, double*, double*, int, hipStream_t, int);
 template hipError_t rows_phase_adj<double>(void*, const void*, int, int, int, const PhaseParams&,
                                            double, const double*, const double*, hipStream_t);
 template hipError_t rows_phase_adj<float>(void*, const void*, int, int, int, const PhaseParams&,

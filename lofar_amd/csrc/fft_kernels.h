@@ -380,12 +380,15 @@ hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx,
 template <typename Real>
 // occ (optional, forward transforms only): per-tile occupancy bits from
 // grid_tiles; rows of empty tiles are read as zeros
+// sm_log >= 0: the planes are strip-major, [nx/L][ny][L] with L = 2^sm_log columns
 hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx, int ny, int nw,
-                    double scale, hipStream_t st, const unsigned long long* occ = nullptr);
+                    double scale, hipStream_t st, const unsigned long long* occ = nullptr,
+                    int sm_log = -1);
 template <typename Real>
+// sm_log >= 0: the planes are strip-major with strips of 2^sm_log columns
 hipError_t rows_phase(const void* cube, const void* tw, int nx, int ny, int nw,
                       const PhaseParams& pp, double scale, double* re, double* im, int accumulate,
-                      hipStream_t st);
+                      hipStream_t st, int sm_log = -1);
 template <typename Real>
 hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
                           const PhaseParams& pp, double scale, const double* re, const double* im,

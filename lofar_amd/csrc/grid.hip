@@ -58,6 +58,35 @@ __device__ __forceinline__ Win32 decode32(const Entry32& en, const Geometry& g, 
     return w;
 }
 
+// Cell c of a tile's flush, ordered so that consecutive c are stored at
+// consecutive addresses: row-major planes, row c/64 and column c%64;
+// strip-major planes ([nx/L][ny][L], L = 2^sm_log), strip by strip, each strip's
+// 64 rows of L columns.  a is the cell's index in the LDS tile.
+struct FlushCell {
+    int row, col, a;
+    long long off;
+};
+__device__ __forceinline__ FlushCell flush_cell(int c, int sm_log, const Geometry& g, long long w,
+                                                int u0, int v0)
+{
+    FlushCell f;
+    if (sm_log < 0) {
+        f.row = c / kTile;
+        f.col = c % kTile;
+        f.off = (w * g.ny + v0 + f.row) * (long long)g.nx + u0 + f.col;
+    } else {
+        const int L = 1 << sm_log;
+        const int s = c / (kTile * L), rem = c % (kTile * L);
+        f.row = rem >> sm_log;
+        f.col = s * L + (rem & (L - 1));
+        const int u = u0 + f.col, v = v0 + f.row;
+        f.off = w * (long long)g.ny * g.nx + ((((long long)(u >> sm_log)) * g.ny + v) << sm_log) +
+                (u & (L - 1));
+    }
+    f.a = f.row * kTile + f.col;
+    return f;
+}
+
 // LDS tiles hold the real parts of the 64x64 cells, then the imaginary parts:
 // an atomic instruction's lanes (random cells) then spread over all banks.
 constexpr int kPlane = kTile * kTile;
@@ -69,7 +98,8 @@ constexpr int kPlane = kTile * kTile;
 template <int WIN, bool FP64, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const long long* offsets,
                                                     const void* entries, void* cube,
-                                                    int accumulate, unsigned long long* occ)
+                                                    int accumulate, unsigned long long* occ,
+                                                    int sm_log)
 {
     __shared__ double acc[2 * kTile * kTile];
     const long long bkt = blockIdx.x;
@@ -92,10 +122,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
     }
     if (beg == end) {
         if (!accumulate) {
-            for (int c = tid; c < kTile * kTile; c += NT) {
-                const int row = c / kTile, col = c % kTile;
-                if (row < trows && col < tcols) {
-                    const long long o = cell0 + (long long)row * g.nx + col;
+            for (int c = tid; c < kPlane; c += NT) {
+                const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+                if (f.row < trows && f.col < tcols) {
+                    const long long o = f.off;
                     if constexpr (FP64)
                         ((double2*)cube)[o] = double2{0.0, 0.0};
                     else
@@ -184,14 +214,13 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
         }
     }
     __syncthreads();
-    for (int c = tid; c < kTile * kTile; c += NT) {
-        const int row = c / kTile, col = c % kTile;
-        if (row < trows && col < tcols) {
-            const long long o = cell0 + (long long)row * g.nx + col;
+    for (int c = tid; c < kPlane; c += NT) {
+        const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+        if (f.row < trows && f.col < tcols) {
             if constexpr (FP64)
-                ((double2*)cube)[o] = double2{acc[c], acc[kPlane + c]};
+                ((double2*)cube)[f.off] = double2{acc[f.a], acc[kPlane + f.a]};
             else
-                ((float2*)cube)[o] = float2{(float)acc[c], (float)acc[kPlane + c]};
+                ((float2*)cube)[f.off] = float2{(float)acc[f.a], (float)acc[kPlane + f.a]};
         }
     }
 }
@@ -227,7 +256,7 @@ __device__ __forceinline__ unsigned long long to_fixed(double y)
 template <int WIN, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, const long long* offsets,
                                                        const Entry32* entries, float2* cube,
-                                                       unsigned long long* occ)
+                                                       unsigned long long* occ, int sm_log)
 {
     __shared__ unsigned long long acc[2 * kTile * kTile];
     __shared__ float wred[NT / 64];
@@ -238,7 +267,6 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     const int u0 = tu * kTile, v0 = tv * kTile;
     const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
-    const long long cell0 = (w * g.ny + v0) * (long long)g.nx + u0;
     const int tid = threadIdx.x;
     const double norm = 0.5 / LOFAR_REF_PI;
 
@@ -251,10 +279,9 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
             atomicOr(&occ[(w * g.ntu + tu) * ((g.ntv + 63) / 64) + tv / 64], 1ull << (tv % 64));
     }
     if (beg == end) {
-        for (int c = tid; c < kTile * kTile; c += NT) {
-            const int row = c / kTile, col = c % kTile;
-            if (row < trows && col < tcols)
-                cube[cell0 + (long long)row * g.nx + col] = float2{0.0f, 0.0f};
+        for (int c = tid; c < kPlane; c += NT) {
+            const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+            if (f.row < trows && f.col < tcols) cube[f.off] = float2{0.0f, 0.0f};
         }
         return;
     }
@@ -311,11 +338,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
             }
         }
         __syncthreads();
-        for (int c = tid; c < kTile * kTile; c += NT) {
-            const int row = c / kTile, col = c % kTile;
-            if (row < trows && col < tcols)
-                cube[cell0 + (long long)row * g.nx + col] =
-                    float2{(float)dacc[c], (float)dacc[kPlane + c]};
+        for (int c = tid; c < kPlane; c += NT) {
+            const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+            if (f.row < trows && f.col < tcols)
+                cube[f.off] = float2{(float)dacc[f.a], (float)dacc[kPlane + f.a]};
         }
         return;
     }
@@ -356,12 +382,12 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     }
     __syncthreads();
     const double inv = ldexp(1.0, -S);
-    for (int c = tid; c < kTile * kTile; c += NT) {
-        const int row = c / kTile, col = c % kTile;
-        if (row < trows && col < tcols) {
-            const double re = (double)(long long)acc[c] * inv;
-            const double im = (double)(long long)acc[kPlane + c] * inv;
-            cube[cell0 + (long long)row * g.nx + col] = float2{(float)re, (float)im};
+    for (int c = tid; c < kPlane; c += NT) {
+        const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+        if (f.row < trows && f.col < tcols) {
+            const double re = (double)(long long)acc[f.a] * inv;
+            const double im = (double)(long long)acc[kPlane + f.a] * inv;
+            cube[f.off] = float2{(float)re, (float)im};
         }
     }
 }
@@ -389,7 +415,7 @@ template <int WIN, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
                                                          const long long* offsets,
                                                          const Entry64* entries, double2* cube,
-                                                         unsigned long long* occ)
+                                                         unsigned long long* occ, int sm_log)
 {
     __shared__ unsigned long long acc[2 * kPlane];
     __shared__ double wred[NT / 64];
@@ -401,7 +427,6 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     const int u0 = tu * kTile, v0 = tv * kTile;
     const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
-    const long long cell0 = (w * g.ny + v0) * (long long)g.nx + u0;
     const int tid = threadIdx.x;
     const double norm = 0.5 / LOFAR_REF_PI;
 
@@ -412,9 +437,8 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     }
     if (beg == end) {
         for (int c = tid; c < kPlane; c += NT) {
-            const int row = c / kTile, col = c % kTile;
-            if (row < trows && col < tcols)
-                cube[cell0 + (long long)row * g.nx + col] = double2{0.0, 0.0};
+            const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+            if (f.row < trows && f.col < tcols) cube[f.off] = double2{0.0, 0.0};
         }
         return;
     }
@@ -468,9 +492,9 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
         }
         __syncthreads();
         for (int c = tid; c < kPlane; c += NT) {
-            const int row = c / kTile, col = c % kTile;
-            if (row < trows && col < tcols)
-                cube[cell0 + (long long)row * g.nx + col] = double2{dacc[c], dacc[kPlane + c]};
+            const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+            if (f.row < trows && f.col < tcols)
+                cube[f.off] = double2{dacc[f.a], dacc[kPlane + f.a]};
         }
         return;
     }
@@ -511,10 +535,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     __syncthreads();
     const double inv = ldexp(1.0, -S);
     for (int c = tid; c < kPlane; c += NT) {
-        const int row = c / kTile, col = c % kTile;
-        if (row < trows && col < tcols)
-            cube[cell0 + (long long)row * g.nx + col] =
-                double2{(double)(long long)acc[c] * inv, (double)(long long)acc[kPlane + c] * inv};
+        const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
+        if (f.row < trows && f.col < tcols)
+            cube[f.off] = double2{(double)(long long)acc[f.a] * inv,
+                                  (double)(long long)acc[kPlane + f.a] * inv};
     }
 }
 
@@ -638,44 +662,50 @@ static int grid_nt()
 
 template <int WIN, int NT>
 static void launch_grid_nt(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                           void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
+                           void* cube, int accumulate, unsigned long long* occ, int sm_log,
+                           hipStream_t st)
 {
     const dim3 grid((unsigned)b.map.nbuckets());
     if (precision == LOFAR_FP64 && !accumulate && grid_cfg() == 0)
         hipLaunchKernelGGL((k_grid_tiles_fx64<WIN, NT>), grid, dim3(NT), 0, st, g, b.map,
-                           b.offsets, (const Entry64*)entries, (double2*)cube, occ);
+                           b.offsets, (const Entry64*)entries, (double2*)cube, occ, sm_log);
     else if (precision == LOFAR_FP64)
         hipLaunchKernelGGL((k_grid_tiles<WIN, true, NT>), grid, dim3(NT), 0, st, g, b.map,
-                           b.offsets, entries, cube, accumulate, occ);
+                           b.offsets, entries, cube, accumulate, occ, sm_log);
     else if (!accumulate && grid_cfg() == 0)
         hipLaunchKernelGGL((k_grid_tiles_fx<WIN, NT>), grid, dim3(NT), 0, st, g, b.map, b.offsets,
-                           (const Entry32*)entries, (float2*)cube, occ);
+                           (const Entry32*)entries, (float2*)cube, occ, sm_log);
     else
         hipLaunchKernelGGL((k_grid_tiles<WIN, false, NT>), grid, dim3(NT), 0, st, g, b.map,
-                           b.offsets, entries, cube, accumulate, occ);
+                           b.offsets, entries, cube, accumulate, occ, sm_log);
 }
 
 template <int WIN>
 static void launch_grid(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                        void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
+                        void* cube, int accumulate, unsigned long long* occ, int sm_log,
+                        hipStream_t st)
 {
     switch (grid_nt()) {
-        case 1024: return launch_grid_nt<WIN, 1024>(precision, g, b, entries, cube, accumulate, occ, st);
-        case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, occ, st);
-        default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, occ, st);
+        case 1024: return launch_grid_nt<WIN, 1024>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
+        case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
+        default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     }
 }
 
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                      void* cube, int accumulate, unsigned long long* occ, hipStream_t st)
+                      void* cube, int accumulate, unsigned long long* occ, int sm_log,
+                      hipStream_t st)
 {
+    // strip-major flushes need whole strips no wider than a tile
+    if (sm_log >= 0 && (accumulate || (1 << sm_log) > kTile || g.nx % (1 << sm_log) != 0))
+        return hipErrorInvalidValue;
     const int win = 2 * g.K + 1;
     if (win <= 7)
-        launch_grid<7>(precision, g, b, entries, cube, accumulate, occ, st);
+        launch_grid<7>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     else if (win <= 9)
-        launch_grid<9>(precision, g, b, entries, cube, accumulate, occ, st);
+        launch_grid<9>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     else if (win <= 17)
-        launch_grid<17>(precision, g, b, entries, cube, accumulate, occ, st);
+        launch_grid<17>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -140,8 +140,11 @@ hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, con
 // occ (optional, nw*ntu*ceil(ntv/64) words, zeroed by the caller): bit tv%64 of
 // word (w, tu, tv/64) is set for every tile holding samples; empty tiles are then
 // left unwritten (for a consumer that reads them as zeros).
+// sm_log >= 0 (without accumulate): the cube is written strip-major, each plane
+// [nx/L][ny][L] with L = 2^sm_log columns per strip.
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
-                      void* cube, int accumulate, unsigned long long* occ, hipStream_t st);
+                      void* cube, int accumulate, unsigned long long* occ, int sm_log,
+                      hipStream_t st);
 // fills n words with v (stream-ordered, no memset node)
 hipError_t fill_u32(unsigned* p, long long n, unsigned v, hipStream_t st);
 hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, const DEntry* entries,

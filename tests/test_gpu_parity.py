@@ -353,6 +353,41 @@ def test_image_ignores_stale_scratch_cube(cuda, precision, tol):
         assert rel(H(r) + 1j * H(i), want) < tol
 
 
+_LAYOUT_SCRIPT = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle as O
+from lofar_amd import imager
+from lofar_amd.synth import make_obs
+from tests.test_gpu_parity import dev_inputs, rel, H
+dev = torch.device("cuda:0")
+for n, nx, nw in ((40000, 256, 4), (5000, 64, 3)):
+    d = make_obs(n, nx, nw, seed=nx + nw)
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
+                   nx, nx, d["resolution"], d["wmin"], d["wmax"])
+    for prec, tol in (("fp64", 1e-10), ("fp32c", 1e-6)):
+        cube = torch.full((nw, nx, nx), float("nan"), dtype=imager.cube_dtype(prec), device=dev)
+        r, i = imager.image(*dev_inputs(d, dev), nw, 7, nx, nx, d["resolution"], d["wmin"],
+                            d["wmax"], precision=prec, cube=cube)
+        e = rel(H(r) + 1j * H(i), want)
+        assert e < tol, (n, nx, prec, e)
+print("LAYOUT_OK")
+"""
+
+
+@pytest.mark.parametrize("sm_log", ["-1", "2", "3", "4", "6"])
+def test_image_scratch_layouts(cuda, sm_log):
+    """The fused path's scratch cube in row-major or strip-major planes of 2^sm_log
+    columns (LOFAR_SM_LOG, read once per process) gives the same image."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, LOFAR_SM_LOG=sm_log)
+    r = subprocess.run([sys.executable, "-c", _LAYOUT_SCRIPT, root], env=env, cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "LAYOUT_OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
 def test_dropin_chain_matches_oracle(cuda):
     # wstack -> fftw_data -> phase_correction through the host drop-ins, as gridding.c,
     # fourier_transform.c:7-74 and :173 call them
