@@ -218,7 +218,7 @@ BucketCfg bucket_cfg()
 {
     static BucketCfg cfg{0, 0, 0};
     if (!cfg.sst) {
-        int a = 16;
+        int a = 8;
         long long b = 16384, l2 = 8192;
         if (const char* e = getenv("LOFAR_BUCKET_CFG")) sscanf(e, "%d,%lld,%lld", &a, &b, &l2);
         if (a < 1 || a > 16 || (a & (a - 1))) a = 16;
