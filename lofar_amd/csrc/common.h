@@ -65,4 +65,28 @@ struct alignas(32) DEntry {
     long long pad;
 };
 
+// Workgroup barrier that orders LDS accesses only: it waits for this wave's LDS
+// operations (lgkmcnt 0) but not for its global memory operations, so global
+// loads (direct-to-LDS loads into another buffer among them) and stores stay in
+// flight across it.  The empty asm statements keep
+// the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // gfx9 encoding: lgkmcnt(0), vmcnt/expcnt at max
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Wait until this wave's global loads (including direct-to-LDS loads) are done.
+__device__ __forceinline__ void wait_loads()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), lgkmcnt/expcnt at max
+    asm volatile("" ::: "memory");
+}
+
+// number of compute units of the current device (cached; persistent-grid sizing)
+int device_cus();
+
 }  // namespace lofar

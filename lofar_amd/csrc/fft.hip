@@ -188,7 +188,46 @@ struct PixelPhase<false> {
 // written once.  The thread's twiddle factors are loaded into registers once
 // (TWREG) since every plane's row uses the same ones.  FP64 phase and
 // accumulation in both precision modes.
-template <typename Real, int N, int E, bool RECUR, bool TWREG>
+// Row prefetch for k_rows_phase: row iv of plane psrc goes straight into
+// pbuf[0, N) with 16-byte direct-to-LDS loads (global_load_lds_dwordx4); wave
+// instruction c of the row writes 1 KiB, elements [c*K, (c+1)*K) with K = 1024 /
+// sizeof(T2) (one run of a row-major row, or of one strip of a strip-major plane
+// when its strips are at least K elements wide).
+template <typename T2, int N, int TPL>
+__device__ __forceinline__ void prefetch_row(const T2* psrc, T2* pbuf, int t, int iv, int ny,
+                                             int sm_log)
+{
+    constexpr int PER = 16 / (int)sizeof(T2);  // elements per lane
+    constexpr int K = 64 * PER;                 // elements per wave instruction
+    constexpr int NWAVE = TPL / 64;
+    static_assert(NWAVE >= 1 && N % K == 0, "whole waves, whole 1 KiB runs");
+    const int lane = t & 63, wave = t >> 6;
+    constexpr int ITER = (N / K + NWAVE - 1) / NWAVE;
+    // wave-uniform run start (scalar registers) plus a per-lane offset
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    const T2* lane_src = psrc + lane * PER;
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        const int c = wave_u + i * NWAVE;
+        if (c >= N / K) break;
+        const int u = c * K;  // the run never crosses a strip: strips are >= K wide
+        const long long o = sm_log < 0 ? (long long)iv * N + u
+                                        : ((((long long)(u >> sm_log) * ny + iv) << sm_log) |
+                                           (u & ((1 << sm_log) - 1)));
+        __builtin_amdgcn_global_load_lds((const void*)(lane_src + o),
+                                         (__attribute__((address_space(3))) void*)(pbuf + u), 16,
+                                         0, 0);
+    }
+}
+
+// LDS bytes of k_rows_phase with the row prefetch buffer
+template <typename Real, int N, int E>
+constexpr bool rows_prefetch_fits()
+{
+    return (Plan<N, E>::LDS + N) * (int)sizeof(C2<Real>) <= 160 * 1024;
+}
+
+template <typename Real, int N, int E, bool RECUR, bool TWREG, bool PF>
 __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>* cube,
                                                                 const C2<Real>* tw, int ny, int nw,
                                                                 PhaseParams pp, double scale,
@@ -199,6 +238,8 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
     using P = Plan<N, E>;
     constexpr int NTW = tw_count<N, E>(1);
     __shared__ T2 lds[P::LDS];
+    // PF: plane iw+1's row is loaded into pbuf while plane iw is transformed
+    __shared__ __align__(16) T2 pbuf[PF ? N : 1];
     const int iv = blockIdx.x;
     const int t = threadIdx.x;
     const long long img_row = (long long)iv * N;
@@ -223,13 +264,25 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
                             : ((((long long)(u >> sm_log) * ny + iv) << sm_log) |
                                (u & ((1 << sm_log) - 1)));
     }
+    if constexpr (PF) prefetch_row<T2, N, P::TPL>(cube, pbuf, t, iv, ny, sm_log);
     T2 v[P::E];
     for (int iw = 0; iw < nw; iw++) {
-        const T2* psrc = cube + (long long)iw * plane_stride;
+        if constexpr (PF) {
+            wait_loads();
+            lds_barrier();  // every wave's part of the row is in pbuf
 #pragma unroll
-        for (int q = 0; q < P::E; q++) v[q] = psrc[off[q]];
+            for (int q = 0; q < P::E; q++) v[q] = pbuf[P::in_index(t, q)];
+            lds_barrier();  // pbuf read by every wave: free for the next plane
+            if (iw + 1 < nw)
+                prefetch_row<T2, N, P::TPL>(cube + (long long)(iw + 1) * plane_stride, pbuf, t, iv,
+                                            ny, sm_log);
+        } else {
+            const T2* psrc = cube + (long long)iw * plane_stride;
+#pragma unroll
+            for (int q = 0; q < P::E; q++) v[q] = psrc[off[q]];
+        }
         if constexpr (TWREG)
-            stages_rt<T2, N, E, 1, 1>(v, lds, t, twr);
+            stages_rt<T2, N, E, 1, 1, PF>(v, lds, t, twr);
         else
             stages<T2, N, E, 1, 1>(v, lds, 1, t, tw);
         const double c2w = 2.0 * LOFAR_REF_PI * (pp.wmin + iw * pp.dw);
@@ -403,20 +456,29 @@ static int phase_cfg()
     return cfg;
 }
 
-template <typename Real, int N, int E, bool RECUR, bool TWREG>
+template <typename Real, int N, int E, bool RECUR, bool TWREG, bool PF = false>
 static hipError_t launch_rows_phase_cfg(const void* cube, const void* tw, int ny, int nw,
                                         const PhaseParams& pp, double scale, double* re,
                                         double* im, int accumulate, int sm_log, hipStream_t st)
 {
     using T2 = C2<Real>;
-    hipLaunchKernelGGL((k_rows_phase<Real, N, E, RECUR, TWREG>), dim3(ny), dim3(Plan<N, E>::TPL),
-                       0, st, (const T2*)cube, (const T2*)tw, ny, nw, pp, scale, re, im,
-                       accumulate, sm_log);
+    if constexpr (PF) {
+        // the prefetch reads whole 1 KiB runs: row-major rows, or strips at least
+        // that wide
+        constexpr int K = 1024 / (int)sizeof(T2);
+        if (sm_log >= 0 && (1 << sm_log) < K)
+            return launch_rows_phase_cfg<Real, N, E, RECUR, TWREG, false>(
+                cube, tw, ny, nw, pp, scale, re, im, accumulate, sm_log, st);
+    }
+    hipLaunchKernelGGL((k_rows_phase<Real, N, E, RECUR, TWREG, PF>), dim3(ny),
+                       dim3(Plan<N, E>::TPL), 0, st, (const T2*)cube, (const T2*)tw, ny, nw, pp,
+                       scale, re, im, accumulate, sm_log);
     return hipGetLastError();
 }
 
 // fused row-pass configurations (elements per thread, phase recurrence or direct
-// sincos, twiddles in registers); entry 0 is the default
+// sincos, twiddles in registers, next row prefetched into LDS); entry 0 is the
+// default
 template <typename Real, int N>
 static hipError_t launch_rows_phase_n(const void* cube, const void* tw, int ny, int nw,
                                       const PhaseParams& pp, double scale, double* re, double* im,
@@ -428,7 +490,11 @@ static hipError_t launch_rows_phase_n(const void* cube, const void* tw, int ny, 
         case 2: return launch_rows_phase_cfg<Real, N, 8, true, false>(ARGS_);
         case 3: return launch_rows_phase_cfg<Real, N, 16, true, true>(ARGS_);
         case 4: return launch_rows_phase_cfg<Real, N, 4, true, true>(ARGS_);
-        default: return launch_rows_phase_cfg<Real, N, 8, true, true>(ARGS_);
+        case 5: return launch_rows_phase_cfg<Real, N, 8, true, true>(ARGS_);
+        default:
+            return launch_rows_phase_cfg<Real, N, 8, true, true,
+                                         (Plan<N, 8>::TPL >= 64 &&
+                                          rows_prefetch_fits<Real, N, 8>())>(ARGS_);
     }
 #undef ARGS_
 }

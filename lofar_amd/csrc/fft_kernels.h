@@ -283,8 +283,18 @@ __device__ __forceinline__ void load_twiddles(T2* twr, int t, const T2* __restri
         load_twiddles<T2, N, EREQ, NS * R>(twr + (NS > 1 ? M * (R - 1) : 0), t, tw);
 }
 
+template <bool LB>
+__device__ __forceinline__ void stage_barrier()
+{
+    if constexpr (LB)
+        lds_barrier();
+    else
+        __syncthreads();
+}
+
 // stages() with the twiddle factors taken from registers filled by load_twiddles().
-template <typename T2, int N, int EREQ, int SIGN, int NS>
+// LB: exchange barriers wait for LDS operations only (lds_barrier).
+template <typename T2, int N, int EREQ, int SIGN, int NS, bool LB = false>
 __device__ __forceinline__ void stages_rt(T2* v, T2* lds, int t, const T2* twr)
 {
     using P = Plan<N, EREQ>;
@@ -305,7 +315,7 @@ __device__ __forceinline__ void stages_rt(T2* v, T2* lds, int t, const T2* twr)
         constexpr int NS2 = NS * R;
         constexpr int R2 = (N / NS2 >= P::RX) ? P::RX : N / NS2;
         constexpr int M2 = E / R2;
-        __syncthreads();
+        stage_barrier<LB>();
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int b = t + m * TPL;
@@ -314,14 +324,14 @@ __device__ __forceinline__ void stages_rt(T2* v, T2* lds, int t, const T2* twr)
 #pragma unroll
             for (int q = 0; q < R; q++) lds[P::pad(base + q * NS)] = v[m * R + q];
         }
-        __syncthreads();
+        stage_barrier<LB>();
 #pragma unroll
         for (int m = 0; m < M2; m++) {
             const int b = t + m * TPL;
 #pragma unroll
             for (int q = 0; q < R2; q++) v[m * R2 + q] = lds[P::pad(b + q * (N / R2))];
         }
-        stages_rt<T2, N, EREQ, SIGN, NS2>(v, lds, t, twr + (NS > 1 ? M * (R - 1) : 0));
+        stages_rt<T2, N, EREQ, SIGN, NS2, LB>(v, lds, t, twr + (NS > 1 ? M * (R - 1) : 0));
     }
 }
 
@@ -394,7 +404,5 @@ hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
                           const PhaseParams& pp, double scale, const double* re, const double* im,
                           hipStream_t st);
 
-// number of compute units of the current device (cached; persistent-grid sizing)
-int device_cus();
 
 }  // namespace lofar

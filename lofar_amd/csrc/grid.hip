@@ -2,13 +2,16 @@
 //
 // Gridding (wstack(), w-stacking_omp.c:25-145) is done owner-computes: every
 // (w-plane, 64x64 uv tile) is one workgroup that accumulates all samples whose
-// Gaussian window touches the tile in an LDS tile of FP64 accumulators
-// (ds_add_f64), then writes the tile to HBM once with plain coalesced stores.
+// Gaussian window touches the tile in an LDS tile (64-bit fixed point with
+// ds_add_u64 in both precision modes, ds_add_f64 for tiles holding non-finite
+// values and for accumulate mode), then writes the tile to HBM once with plain
+// coalesced stores.
 // A sample whose window straddles tiles is listed in each of them with the
 // window clipped to that tile, so no two workgroups write the same cell and no
 // global atomics or zero-fill pass are needed.  The (w-plane, tile) buckets
 // come from the two-level counting sort in bucket.hip; a bucket index that
 // names a tile beyond the grid (a partial super-tile) ends its workgroup at once.
+#include <algorithm>
 #include <cstdlib>
 
 #include "grid_kernels.h"
@@ -76,7 +79,8 @@ __device__ __forceinline__ FlushCell flush_cell(int c, int sm_log, const Geometr
         f.off = (w * g.ny + v0 + f.row) * (long long)g.nx + u0 + f.col;
     } else {
         const int L = 1 << sm_log;
-        const int s = c / (kTile * L), rem = c % (kTile * L);
+        static_assert(kTile == 64, "strip indexing assumes 64-cell tiles");
+        const int s = c >> (6 + sm_log), rem = c & ((kTile << sm_log) - 1);
         f.row = rem >> sm_log;
         f.col = s * L + (rem & (L - 1));
         const int u = u0 + f.col, v = v0 + f.row;
@@ -253,13 +257,48 @@ __device__ __forceinline__ unsigned long long to_fixed(double y)
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// One FP32C entry into the fixed-point tile (real parts, then imaginary parts
+// kPlane words on): the cells of its clipped window.
+template <int WIN>
+__device__ __forceinline__ void accumulate_fx32(unsigned long long* acc, const Geometry& g, int u0,
+                                                int v0, double norm, double scale,
+                                                const Entry32& en)
+{
+    const Win32 w32 = decode32(en, g, u0, v0);
+    const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
+    unsigned long long* base = &acc[b0 * kTile + a0];
+    double eu[WIN];
+#pragma unroll
+    for (int q = 0; q < WIN; q++) {
+        const float du = (float)(w32.iu + q) + w32.hu;
+        eu[q] = (q <= na) ? (double)expf(-(du * du) * 0.5f) : 0.0;
+    }
+#pragma unroll
+    for (int kq = 0; kq < WIN; kq++) {
+        if (kq <= nb) {
+            const float dv = (float)(w32.iv + kq) + w32.hv;
+            const float f = (float)norm * expf(-(dv * dv) * 0.5f);
+            const double fr = (double)(f * en.vr) * scale, fi = (double)(f * en.vi) * scale;
+            unsigned long long* row = base + kq * kTile;
+#pragma unroll
+            for (int q = 0; q < WIN; q++) {
+                if (q <= na) {
+                    atomicAdd(row + q, to_fixed(eu[q] * fr));
+                    atomicAdd(row + kPlane + q, to_fixed(eu[q] * fi));
+                }
+            }
+        }
+    }
+}
+
 template <int WIN, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, const long long* offsets,
                                                        const Entry32* entries, float2* cube,
                                                        unsigned long long* occ, int sm_log)
 {
-    __shared__ unsigned long long acc[2 * kTile * kTile];
+    __shared__ __align__(16) unsigned long long acc[2 * kPlane];
     __shared__ float wred[NT / 64];
+    __shared__ int wfin[NT / 64];
     const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
@@ -286,26 +325,35 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
         return;
     }
     // largest |vis| of the tile (NaN real parts are skipped upstream; any other
-    // non-finite value sends the tile down the floating-point path)
-    float m = 0.0f;
-    bool finite = true;
-    for (long long e = beg + tid; e < end; e += NT) {
+    // non-finite value sends the tile down the floating-point path).  The first
+    // NT entries are loaded once, before the LDS clear, and kept in registers.
+    const long long n = end - beg;
+    const int lane = tid & 63, wave = tid >> 6;
+    Entry32 en0 = {0u, 0u, 0.0f, 0.0f};
+    if (tid < n) en0 = entries[beg + tid];
+    for (int c = tid; c < kPlane; c += NT)  // 2*kPlane words, 16 bytes a store
+        ((ulonglong2*)acc)[c] = ulonglong2{0ull, 0ull};  // also 0.0 as double
+    float m = fmaxf(fabsf(en0.vr), fabsf(en0.vi));
+    bool finite = isfinite(en0.vr) && isfinite(en0.vi);
+    for (long long e = beg + NT + tid; e < end; e += NT) {
         const Entry32 en = entries[e];
         finite = finite && isfinite(en.vr) && isfinite(en.vi);
         m = fmaxf(m, fmaxf(fabsf(en.vr), fabsf(en.vi)));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    const int lane = tid & 63, wave = tid >> 6;
-    if (lane == 0) wred[wave] = m;
-    __shared__ int all_finite;
-    if (tid == 0) all_finite = 1;
+    const bool wave_finite = __all(finite);
+    if (lane == 0) {
+        wred[wave] = m;
+        wfin[wave] = wave_finite ? 1 : 0;
+    }
     __syncthreads();
-    if (!finite) all_finite = 0;
     m = wred[0];
+    bool all_finite = wfin[0] != 0;
 #pragma unroll
-    for (int i = 1; i < NT / 64; i++) m = fmaxf(m, wred[i]);
-    for (int c = tid; c < 2 * kTile * kTile; c += NT) acc[c] = 0ull;  // also 0.0 as double
-    __syncthreads();
+    for (int i = 1; i < NT / 64; i++) {
+        m = fmaxf(m, wred[i]);
+        all_finite = all_finite && wfin[i] != 0;
+    }
     if (!all_finite) {
         // floating-point accumulation (ds_add_f64), as k_grid_tiles<WIN, false>
         double* dacc = (double*)acc;
@@ -348,38 +396,13 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     int ex = 0;
     frexp(norm * (double)(m > 0.0f ? m : 1.0f), &ex);  // norm*m < 2^ex
     int hb = 2;
-    for (long long cnt = end - beg; cnt > 1; cnt >>= 1) hb++;
+    for (long long cnt = n; cnt > 1; cnt >>= 1) hb++;
     const int S = kFixedBits - hb - ex;
     const double scale = ldexp(1.0, S);
 
-    for (long long e = beg + tid; e < end; e += NT) {
-        const Entry32 en = entries[e];
-        const Win32 w32 = decode32(en, g, u0, v0);
-        const int a0 = w32.t.a0, na = w32.t.na, b0 = w32.t.b0, nb = w32.t.nb;
-        unsigned long long* base = &acc[b0 * kTile + a0];
-        double eu[WIN];
-#pragma unroll
-        for (int q = 0; q < WIN; q++) {
-            const float du = (float)(w32.iu + q) + w32.hu;
-            eu[q] = (q <= na) ? (double)expf(-(du * du) * 0.5f) : 0.0;
-        }
-#pragma unroll
-        for (int kq = 0; kq < WIN; kq++) {
-            if (kq <= nb) {
-                const float dv = (float)(w32.iv + kq) + w32.hv;
-                const float f = (float)norm * expf(-(dv * dv) * 0.5f);
-                const double fr = (double)(f * en.vr) * scale, fi = (double)(f * en.vi) * scale;
-                unsigned long long* row = base + kq * kTile;
-#pragma unroll
-                for (int q = 0; q < WIN; q++) {
-                    if (q <= na) {
-                        atomicAdd(row + q, to_fixed(eu[q] * fr));
-                        atomicAdd(row + kPlane + q, to_fixed(eu[q] * fi));
-                    }
-                }
-            }
-        }
-    }
+    if (tid < n) accumulate_fx32<WIN>(acc, g, u0, v0, norm, scale, en0);
+    for (long long e = beg + NT + tid; e < end; e += NT)
+        accumulate_fx32<WIN>(acc, g, u0, v0, norm, scale, entries[e]);
     __syncthreads();
     const double inv = ldexp(1.0, -S);
     for (int c = tid; c < kPlane; c += NT) {
@@ -394,32 +417,147 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
 
 // FP64 gridding with exact fixed-point accumulation: as k_grid_tiles_fx, with
 // double visibilities and a 62-bit range (S = 62 - hb - ex); every contribution
-// eu*fr is the double product of the reference's factors, rounded to the
-// nearest multiple of 2^-S, and the integer sums are exact and independent of
-// order.  The quantum is ~2^-(60-log2(entries)) of the tile's largest
+// eu * (norm*ev*vis*2^S) is rounded once, from the exact product, to the nearest
+// integer (fixed62_of_product), and the integer sums are exact and independent
+// of order.  The quantum is ~2^-(60-log2(entries)) of the tile's largest
 // contribution and the rounding is unbiased, so the cells agree with a
-// double-precision sum to ~1e-15 of the tile's largest contribution.
+// double-precision sum to ~1e-15 of the tile's largest contribution.  The
+// Gaussian factors come from gauss_row (two exponentials per axis).
 constexpr int kFixedBits64 = 62;
 
-// round(y) as a two's-complement 64-bit value, |y| < 2^62: floor(y + 1/2) split
-// into a high and a low 32-bit part, each exact
-__device__ __forceinline__ unsigned long long to_fixed62(double y)
+// round(a*b) (nearest, ties to even, of the exact product) as a two's-complement
+// 64-bit value, |a*b| < 2^62, in three full-rate double operations and one 64-bit
+// shift-add:
+//   t1 = fma(a, b, C1), C1 = 1.5*2^84 - 0x43380000*2^32: t1 stays in [2^84, 2^85),
+//        so ulp(t1) = 2^32 and t1 = C1 + k*2^32 with k = round(a*b / 2^32); the low
+//        word of t1 is k - 0x43380000 (mod 2^32);
+//   c  = (C1 + C2) - t1 = C2 - k*2^32, exact (Sterbenz), C2 = 1.5*2^52;
+//   t2 = fma(a, b, c): ulp(t2) = 1, so t2 = C2 + round(a*b - k*2^32), one rounding;
+//        t2's bits are C2's bits (high word 0x43380000, low word 0) plus that
+//        remainder as a 64-bit integer.
+// (t1 bits << 32) + (t2 bits) = remainder + k*2^32, the high words' 0x43380000
+// cancelling; as k*2^32 has a zero low word this is one 32-bit add to t2's high
+// word (written on a two-word vector so that it stays a 32-bit add).  C1 and
+// C1 + C2 are exact doubles.
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned long long fixed62_of_product(double a, double b)
 {
-    const double t = y + 0.5;
-    const double hi = floor(t * 0x1p-32);
-    const double lo = fma(-hi, 0x1p32, t);
-    return ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)(unsigned)lo;
+    const double t1 = fma(a, b, 0x1.7ffffbcc80000p+84);
+    const double t2 = fma(a, b, 0x1.7ffffbce00000p+84 - t1);
+    u32x2 r = __builtin_bit_cast(u32x2, t2);
+    r.y += __builtin_bit_cast(u32x2, t1).x;
+    return __builtin_bit_cast(unsigned long long, r);
 }
 
+// A fixed-point sum back to double, times 2^-S (exact scaling, one rounding)
+__device__ __forceinline__ double from_fixed(unsigned long long x, double inv)
+{
+    const double hi = (double)(int)(x >> 32), lo = (double)(unsigned)x;
+    return fma(hi, 0x1p32 * inv, lo * inv);
+}
+
+// exp(-(q + 1/2)), q = 0..15, correctly rounded
+__constant__ double kGaussStep[16] = {
+    0x1.368b2fc6f960ap-1, 0x1.c8f87724b5c1dp-3, 0x1.50385c094f425p-4, 0x1.eec1018e4ff66p-6,
+    0x1.6c0504695c417p-7, 0x1.0bd4a5aca7728p-8, 0x1.8a1e18a954ffap-10, 0x1.21f9ba40f31d5p-11,
+    0x1.aab43fb3940a1p-13, 0x1.39f38a5498fe2p-14, 0x1.cdfc263f6a0bap-16, 0x1.53e8d76780b04p-17,
+    0x1.f42ed3f68e690p-19, 0x1.7003974e945dfp-20, 0x1.0ec503d2696e8p-21, 0x1.8e7138077168dp-23};
+
+// Gaussian factors g[q] = exp(-(d0 + q)^2 / 2), q < WIN, from two exponentials:
+// g[q+1] = g[q] * exp(-d0) * exp(-(q + 1/2)); each step adds two roundings, so
+// g[q] is within ~2q ulp of the directly evaluated exponential.
+template <int WIN>
+__device__ __forceinline__ void gauss_row(double d0, double (&gq)[WIN])
+{
+    double x = exp(-(d0 * d0) * 0.5);
+    const double b = exp(-d0);
+    gq[0] = x;
+#pragma unroll
+    for (int q = 1; q < WIN; q++) {
+        x *= b * kGaussStep[q - 1];
+        gq[q] = x;
+    }
+}
+
+// One entry into the fixed-point tile (real parts, then imaginary parts kPlane
+// words on): cell (a, b) of the entry's clipped window gets
+// round(eu[a] * norm * ev[b] * vis * 2^S).  Branch-free: cells of the WIN x WIN
+// square at the window's first cell that lie outside the clipped window get a
+// zero factor, whose fixed-point value is exactly 0, so every lane adds the same
+// cells; the LDS tile is padded so that those addresses stay inside it.
+template <int WIN>
+__device__ __forceinline__ void accumulate_fx64(unsigned long long* acc, const Geometry& g, int u0,
+                                                int v0, double norm, double scale,
+                                                const Entry64& en)
+{
+    const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
+    double eu[WIN], ev[WIN];
+    gauss_row<WIN>((double)(u0 + tw.a0) + 0.5 - en.pu, eu);
+    gauss_row<WIN>((double)(v0 + tw.b0) + 0.5 - en.pv, ev);
+#pragma unroll
+    for (int q = 0; q < WIN; q++) {
+        eu[q] = (q <= tw.na) ? eu[q] : 0.0;
+        ev[q] = (q <= tw.nb) ? ev[q] : 0.0;
+    }
+    unsigned long long* base = &acc[tw.b0 * kTile + tw.a0];
+#pragma unroll
+    for (int kq = 0; kq < WIN; kq++) {
+        const double f = norm * ev[kq];
+        const double fr = f * en.vr * scale, fi = f * en.vi * scale;
+        unsigned long long* row = base + kq * kTile;
+#pragma unroll
+        for (int q = 0; q < WIN; q++) {
+            atomicAdd(row + q, fixed62_of_product(eu[q], fr));
+            atomicAdd(row + kPlane + q, fixed62_of_product(eu[q], fi));
+        }
+    }
+}
+
+// The same entry in floating point (ds_add_f64), for tiles holding a non-finite
+// visibility: direct exponentials, as k_grid_tiles<WIN, true>.
+template <int WIN>
+__device__ __forceinline__ void accumulate_f64(double* acc, const Geometry& g, int u0, int v0,
+                                               double norm, const Entry64& en)
+{
+    const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
+    double eu[WIN];
+#pragma unroll
+    for (int q = 0; q < WIN; q++) {
+        const double du = (double)(u0 + tw.a0 + q) + 0.5 - en.pu;
+        eu[q] = (q <= tw.na) ? exp(-(du * du) * 0.5) : 0.0;
+    }
+    double* base = &acc[tw.b0 * kTile + tw.a0];
+#pragma unroll
+    for (int kq = 0; kq < WIN; kq++) {
+        if (kq <= tw.nb) {
+            const double dv = (double)(v0 + tw.b0 + kq) + 0.5 - en.pv;
+            const double f = norm * exp(-(dv * dv) * 0.5);
+            const double fr = f * en.vr, fi = f * en.vi;
+            double* row = base + kq * kTile;
+#pragma unroll
+            for (int q = 0; q < WIN; q++) {
+                if (q <= tw.na) {
+                    atomicAdd(row + q, eu[q] * fr);
+                    atomicAdd(row + kPlane + q, eu[q] * fi);
+                }
+            }
+        }
+    }
+}
+
+// One 512-thread workgroup per (w-plane, tile), as k_grid_tiles_fx.
 template <int WIN, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
                                                          const long long* offsets,
                                                          const Entry64* entries, double2* cube,
                                                          unsigned long long* occ, int sm_log)
 {
-    __shared__ unsigned long long acc[2 * kPlane];
+    // the WIN x WIN square at a window's first cell reaches (WIN-1)*(kTile+1) words
+    // past the tile's last cell (accumulate_fx64)
+    constexpr int kPad = ((WIN - 1) * (kTile + 1) + 1) / 2 * 2;
+    __shared__ __align__(16) unsigned long long acc[2 * kPlane + kPad];
     __shared__ double wred[NT / 64];
-    __shared__ int all_finite;
+    __shared__ int wfin[NT / 64];
     const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
@@ -442,53 +580,53 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
         }
         return;
     }
-    double m = 0.0;
-    bool finite = true;
-    for (long long e = beg + tid; e < end; e += NT) {
-        const Entry64 en = entries[e];
-        finite = finite && isfinite(en.vr) && isfinite(en.vi);
-        m = fmax(m, fmax(fabs(en.vr), fabs(en.vi)));
+    // Entries are taken in rounds of NT; a round of r entries goes to ceil(r/32)
+    // waves (at most all of them), so that a small tile still spreads over the
+    // SIMDs.  The first round's entry is loaded once, before the LDS clear, and
+    // kept in registers.
+    const long long n = end - beg;
+    const int lane = tid & 63, wave = tid >> 6;
+    auto slot = [&](long long q0) -> long long {
+        const long long r = min((long long)NT, n - q0);
+        const int wu = (int)min((long long)(NT / 64), (r + 31) / 32);
+        const int j = lane * wu + wave;
+        return (wave < wu && j < r) ? q0 + j : -1;
+    };
+    const long long j0 = slot(0);
+    Entry64 en0 = {0.0, 0.0, 0.0, 0.0};
+    if (j0 >= 0) en0 = entries[beg + j0];
+    for (int c = tid; c < kPlane; c += NT)  // 2*kPlane words, 16 bytes a store
+        ((ulonglong2*)acc)[c] = ulonglong2{0ull, 0ull};  // also 0.0 as double
+    double m = fmax(fabs(en0.vr), fabs(en0.vi));
+    bool finite = isfinite(en0.vr) && isfinite(en0.vi);
+    for (long long q0 = NT; q0 < n; q0 += NT) {
+        const long long j = slot(q0);
+        if (j >= 0) {
+            const Entry64 en = entries[beg + j];
+            finite = finite && isfinite(en.vr) && isfinite(en.vi);
+            m = fmax(m, fmax(fabs(en.vr), fabs(en.vi)));
+        }
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    const int lane = tid & 63, wave = tid >> 6;
-    if (lane == 0) wred[wave] = m;
-    if (tid == 0) all_finite = 1;
+    const bool wave_finite = __all(finite);
+    if (lane == 0) {
+        wred[wave] = m;
+        wfin[wave] = wave_finite ? 1 : 0;
+    }
     __syncthreads();
-    if (!finite) all_finite = 0;
     m = wred[0];
+    bool all_finite = wfin[0] != 0;
 #pragma unroll
-    for (int i = 1; i < NT / 64; i++) m = fmax(m, wred[i]);
-    for (int c = tid; c < 2 * kPlane; c += NT) acc[c] = 0ull;  // also 0.0 as double
-    __syncthreads();
+    for (int i = 1; i < NT / 64; i++) {
+        m = fmax(m, wred[i]);
+        all_finite = all_finite && wfin[i] != 0;
+    }
     if (!all_finite) {
-        // floating-point accumulation (ds_add_f64), as k_grid_tiles<WIN, true>
         double* dacc = (double*)acc;
-        for (long long e = beg + tid; e < end; e += NT) {
-            const Entry64 en = entries[e];
-            const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
-            double* base = &dacc[tw.b0 * kTile + tw.a0];
-            double eu[WIN];
-#pragma unroll
-            for (int q = 0; q < WIN; q++) {
-                const double du = (double)(u0 + tw.a0 + q) + 0.5 - en.pu;
-                eu[q] = (q <= tw.na) ? exp(-(du * du) * 0.5) : 0.0;
-            }
-#pragma unroll
-            for (int kq = 0; kq < WIN; kq++) {
-                if (kq <= tw.nb) {
-                    const double dv = (double)(v0 + tw.b0 + kq) + 0.5 - en.pv;
-                    const double f = norm * exp(-(dv * dv) * 0.5);
-                    const double fr = f * en.vr, fi = f * en.vi;
-                    double* row = base + kq * kTile;
-#pragma unroll
-                    for (int q = 0; q < WIN; q++) {
-                        if (q <= tw.na) {
-                            atomicAdd(row + q, eu[q] * fr);
-                            atomicAdd(row + kPlane + q, eu[q] * fi);
-                        }
-                    }
-                }
-            }
+        if (j0 >= 0) accumulate_f64<WIN>(dacc, g, u0, v0, norm, en0);
+        for (long long q0 = NT; q0 < n; q0 += NT) {
+            const long long j = slot(q0);
+            if (j >= 0) accumulate_f64<WIN>(dacc, g, u0, v0, norm, entries[beg + j]);
         }
         __syncthreads();
         for (int c = tid; c < kPlane; c += NT) {
@@ -501,44 +639,21 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     int ex = 0;
     frexp(norm * (m > 0.0 ? m : 1.0), &ex);  // norm*m < 2^ex
     int hb = 2;
-    for (long long cnt = end - beg; cnt > 1; cnt >>= 1) hb++;
+    for (long long cnt = n; cnt > 1; cnt >>= 1) hb++;
     const int S = kFixedBits64 - hb - ex;
     const double scale = ldexp(1.0, S);
 
-    for (long long e = beg + tid; e < end; e += NT) {
-        const Entry64 en = entries[e];
-        const TileWin tw = clip_window((int)en.pu, (int)en.pv, g, u0, v0);
-        unsigned long long* base = &acc[tw.b0 * kTile + tw.a0];
-        double eu[WIN];
-#pragma unroll
-        for (int q = 0; q < WIN; q++) {
-            const double du = (double)(u0 + tw.a0 + q) + 0.5 - en.pu;
-            eu[q] = (q <= tw.na) ? exp(-(du * du) * 0.5) : 0.0;
-        }
-#pragma unroll
-        for (int kq = 0; kq < WIN; kq++) {
-            if (kq <= tw.nb) {
-                const double dv = (double)(v0 + tw.b0 + kq) + 0.5 - en.pv;
-                const double f = norm * exp(-(dv * dv) * 0.5);
-                const double fr = f * en.vr * scale, fi = f * en.vi * scale;
-                unsigned long long* row = base + kq * kTile;
-#pragma unroll
-                for (int q = 0; q < WIN; q++) {
-                    if (q <= tw.na) {
-                        atomicAdd(row + q, to_fixed62(eu[q] * fr));
-                        atomicAdd(row + kPlane + q, to_fixed62(eu[q] * fi));
-                    }
-                }
-            }
-        }
+    if (j0 >= 0) accumulate_fx64<WIN>(acc, g, u0, v0, norm, scale, en0);
+    for (long long q0 = NT; q0 < n; q0 += NT) {
+        const long long j = slot(q0);
+        if (j >= 0) accumulate_fx64<WIN>(acc, g, u0, v0, norm, scale, entries[beg + j]);
     }
     __syncthreads();
     const double inv = ldexp(1.0, -S);
     for (int c = tid; c < kPlane; c += NT) {
         const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
         if (f.row < trows && f.col < tcols)
-            cube[f.off] = double2{(double)(long long)acc[f.a] * inv,
-                                  (double)(long long)acc[kPlane + f.a] * inv};
+            cube[f.off] = double2{from_fixed(acc[f.a], inv), from_fixed(acc[kPlane + f.a], inv)};
     }
 }
 
