@@ -2,8 +2,9 @@
 // two-level counting sort whose histograms live in LDS.
 //
 // Level 1 groups the samples by super-tile (sst x sst tiles of one w-plane):
-// one 32-byte record (position and channel/polarisation-summed visibility,
-// w-stacking_omp.c:85-90,116-133) per super-tile the sample's window touches,
+// one record (position and channel/polarisation-summed visibility,
+// w-stacking_omp.c:85-90,116-133; 32 bytes in FP64 mode, 16 in FP32 mode, Rec32)
+// per super-tile the sample's window touches,
 // which is one record for almost every sample.  Every workgroup takes a
 // contiguous chunk of samples, counts its records (and the tile entries they
 // will expand into) per super-tile in LDS histograms, reserves its range in
@@ -196,13 +197,16 @@ __device__ __forceinline__ void sample_vis(long long i, long long F, long long P
     }
 }
 
+// Rec: Entry64 (FP64 mode) or Rec32 (FP32 mode, position relative to the
+// super-tile, rec_axis)
+template <typename Rec>
 __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long F, long long P,
                                                          const double* uu, const double* vv,
                                                          const double* ww, const float* vr,
                                                          const float* vi, const float* wt,
                                                          Geometry g, TileMap tm, long long chunk,
                                                          const long long* off1,
-                                                         const unsigned* base1, Entry64* rec)
+                                                         const unsigned* base1, Rec* rec)
 {
     extern __shared__ unsigned cur[];
     const int tid = threadIdx.x;
@@ -232,7 +236,17 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
             for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++)
                 for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
                     const long long k = tm.key_st(s[u].gw, sv, su);
-                    rec[off1[k] + atomicAdd(&cur[k], 1u)] = e[u];
+                    const long long at = off1[k] + atomicAdd(&cur[k], 1u);
+                    if constexpr (sizeof(Rec) == sizeof(Rec32)) {
+                        Rec32 r;
+                        r.u = rec_axis(e[u].pu, (su << tm.lsst) * kTile, tm.rfb);
+                        r.v = rec_axis(e[u].pv, (sv << tm.lsst) * kTile, tm.rfb);
+                        r.vr = (float)e[u].vr;
+                        r.vi = (float)e[u].vi;
+                        rec[at] = r;
+                    } else {
+                        rec[at] = e[u];
+                    }
                 }
         }
     }
@@ -273,12 +287,28 @@ enum { kOutFp64 = 0, kOutFp32 = 1, kOutHome = 2 };
 
 // Tiles of the record's window inside super-tile (sv, su), as inclusive ranges
 // of fine coordinates; for the home mode the single home tile.
+template <typename Rec>
+__device__ __forceinline__ void record_pos(const Rec& r, const TileMap&, int, int, double& pu,
+                                           double& pv)
+{
+    pu = r.pu;
+    pv = r.pv;
+}
+__device__ __forceinline__ void record_pos(const Rec32& r, const TileMap& tm, int sv, int su,
+                                           double& pu, double& pv)
+{
+    pu = rec_pos(r.u, (su << tm.lsst) * kTile, tm.rfb);
+    pv = rec_pos(r.v, (sv << tm.lsst) * kTile, tm.rfb);
+}
+
 template <int OUT, typename Rec>
 __device__ __forceinline__ bool record_tiles(const Rec& r, const Geometry& g, const TileMap& tm,
                                              int sv, int su, int& fu_a, int& fu_b, int& fv_a,
                                              int& fv_b, Window& s)
 {
-    s = window_of(r.pu, r.pv, g);
+    double pu, pv;
+    record_pos(r, tm, sv, su, pu, pv);
+    s = window_of(pu, pv, g);
     if constexpr (OUT == kOutHome) {
         int cu, cv;
         home_cell(s, g, cu, cv);
@@ -428,9 +458,11 @@ __global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, cons
                 if constexpr (OUT == kOutFp32) {
                     const int u0 = ((su << tm.lsst) + fu) * kTile;
                     const int v0 = ((sv << tm.lsst) + fv) * kTile;
+                    double pu, pv;
+                    record_pos(x, tm, sv, su, pu, pv);
                     Entry32 e;
-                    e.wu = pack_cell(s.gu - u0, x.pu - (double)s.gu);
-                    e.wv = pack_cell(s.gv - v0, x.pv - (double)s.gv);
+                    e.wu = pack_cell(s.gu - u0, pu - (double)s.gu);
+                    e.wv = pack_cell(s.gv - v0, pv - (double)s.gv);
                     e.vr = (float)x.vr;
                     e.vi = (float)x.vi;
                     ent[p] = e;
@@ -512,13 +544,19 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
                             Entry64* rec, void* entries, hipStream_t st)
 {
-    if (n > 0)
-        hipLaunchKernelGGL(k_l1_scatter_grid, dim3((unsigned)b.nwg), dim3(256),
-                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, F, P, uu, vv, ww, vr, vi,
-                           wt, g, b.map, b.chunk, b.off1, b.base1, rec);
-    if (precision == LOFAR_FP64)
+    if (precision == LOFAR_FP64) {
+        if (n > 0)
+            hipLaunchKernelGGL((k_l1_scatter_grid<Entry64>), dim3((unsigned)b.nwg), dim3(256),
+                               (size_t)b.map.nk1 * sizeof(unsigned), st, n, F, P, uu, vv, ww, vr,
+                               vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec);
         return level2<kOutFp64>(g, b, (const Entry64*)rec, (Entry64*)entries, st);
-    return level2<kOutFp32>(g, b, (const Entry64*)rec, (Entry32*)entries, st);
+    }
+    Rec32* rec32 = (Rec32*)rec;  // the record buffer holds total_rec 32-byte records
+    if (n > 0)
+        hipLaunchKernelGGL((k_l1_scatter_grid<Rec32>), dim3((unsigned)b.nwg), dim3(256),
+                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, F, P, uu, vv, ww, vr, vi,
+                           wt, g, b.map, b.chunk, b.off1, b.base1, rec32);
+    return level2<kOutFp32>(g, b, (const Rec32*)rec32, (Entry32*)entries, st);
 }
 
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,

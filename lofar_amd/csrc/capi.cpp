@@ -273,6 +273,7 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     while ((1 << m.lsst) < sst) m.lsst++;
     m.nstu = (g.ntu + sst - 1) / sst;
     m.nstv = (g.ntv + sst - 1) / sst;
+    m.rfb = rec_fraction_bits(sst);
     m.nk1 = super_tiles(sst);
     if (m.nk1 > kMaxSuperTiles) {
         fprintf(stderr,

@@ -57,6 +57,7 @@ __device__ __forceinline__ void home_cell(const Window& s, const Geometry& g, in
 struct TileMap {
     int sst, lsst;   // super-tile side in tiles (power of two) and its log2
     int nstu, nstv;  // super-tiles along u and v
+    int rfb;         // fraction bits of an Rec32 position (rec_fraction_bits(sst))
     long long nk1;   // super-tile buckets, nw*nstv*nstu
     __host__ __device__ long long key1(int w, int tv, int tu) const
     {
@@ -91,6 +92,46 @@ struct TileMap {
         return tv < g.ntv && tu < g.ntu;
     }
 };
+
+// Level-1 record of the FP32 mode, 16 bytes: per axis the sample's position p
+// relative to its super-tile's origin s0 (first cell), as floor(p) - s0 + 16 in
+// the high 32 - rfb bits and the fraction p - floor(p) in the low rfb bits.  A
+// record only exists for a super-tile its window touches, so floor(p) - s0 lies
+// in [-K-1, sst*64+K), K <= 15.  The fraction is rounded to rfb bits but kept
+// nonzero when it is nonzero and below 1, so the decoded position has the
+// sample's floor and (int) truncation, hence its exact window and tiles; only
+// the offset inside the cell is quantised (2^-rfb cells; rfb = 22 for 8x8-tile
+// super-tiles).  The visibility sums are rounded to float as the FP32 entries
+// hold them.
+struct alignas(16) Rec32 {
+    unsigned u, v;
+    float vr, vi;
+};
+
+// bits left for the fraction when floor(p) - s0 + 16 must fit in [0, sst*64 + 32)
+__host__ __device__ inline int rec_fraction_bits(int sst)
+{
+    int cb = 0;
+    while ((1 << cb) < sst * kTile + 32) cb++;
+    return 32 - cb;
+}
+
+__device__ __forceinline__ unsigned rec_axis(double p, int s0, int fb)
+{
+    const double fl = floor(p);
+    const double f = p - fl;  // [0, 1), exact
+    const long long one = 1LL << fb;
+    long long q = (long long)(f * (double)one + 0.5);
+    if (f > 0.0 && q == 0) q = 1;
+    if (q >= one) q = one - 1;
+    return ((unsigned)((int)fl - s0 + kCellBias) << fb) | (unsigned)q;
+}
+
+__device__ __forceinline__ double rec_pos(unsigned x, int s0, int fb)
+{
+    return (double)((int)(x >> fb) - kCellBias + s0) +
+           (double)(x & ((1u << fb) - 1u)) * ldexp(1.0, -fb);
+}
 
 // A level-2 block: records [beg, end) of super-tile segment k1.
 struct alignas(32) L2Block {
