@@ -571,12 +571,14 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
     CallGuard cg(stream);
     if (cg.status) return cg.status;
     DeviceCtx& c = *cg.c;
-    // FP64: per-tile occupancy bits; the gridder skips empty tiles and the column
-    // pass reads them as zeros (the FP32 column kernel has no registers to spare
-    // for the check at its 1024-thread workgroups, so FP32C writes every tile)
+    // per-tile occupancy bits: the gridder skips empty tiles and the column pass
+    // reads them as zeros (through 32-bit plane offsets, so for planes below 2 GiB;
+    // larger planes are written whole)
     int r = LOFAR_OK;
     unsigned long long* occ = nullptr;
-    if (precision == LOFAR_FP64) {
+    const long long plane_bytes = (long long)grid_size_x * grid_size_y *
+                                  (precision == LOFAR_FP64 ? 16 : 8);
+    if (plane_bytes < (1LL << 31)) {
         const long long ntv = (grid_size_y + kTile - 1) / kTile;
         const long long ntu = (grid_size_x + kTile - 1) / kTile;
         const long long nocc = (long long)num_w_planes * ntu * ((ntv + 63) / 64);
@@ -620,7 +622,7 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
     } else {
         HIPCHK(stage(c, kStCols, cg.st, [&] {
             return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                   1.0, cg.st, nullptr, sm_log);
+                                   1.0, cg.st, occ, sm_log);
         }));
         HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
             return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp, scale,

@@ -76,19 +76,39 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
                      : (active ? u0 + c : 0));
     T2 v[P::E];
     if constexpr (OCC) {
-        // occupancy bits of this strip's tile column (bit tv of word tv/64): rows of
-        // tiles the gridder left unwritten because they hold no samples read as zeros
+        // Loads through a buffer resource over this plane: 32-bit byte offsets, and
+        // an offset past the plane (rows of tiles without samples, columns past the
+        // grid) returns zeros without a memory access.  Planes are below 2 GiB
+        // (launch_cols_cfg checks).
+        const long long plane0 = plane * (long long)N * nx;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(in + plane0), (short)0, (int)((long long)N * nx * (long long)sizeof(T2)),
+            0x00020000);  // raw, range-checked
+        const unsigned col_off = (unsigned)((base - plane0) * (long long)sizeof(T2));
+        const unsigned row_step = (unsigned)(rs * (long long)sizeof(T2));
+        // bit tv of word tv/64: tile row tv of this strip's tile column holds samples
         constexpr int NWD = (N / kTile + 63) / 64;
+        static_assert(NWD <= 2, "occupancy reads cover at most 128 tile rows");
         const int ntu = (nx + kTile - 1) / kTile;
         const unsigned long long* om = occ + (plane * ntu + u0 / kTile) * (NWD > 0 ? NWD : 1);
-        const unsigned long long m0 = om[0];
-        const unsigned long long m1 = NWD > 1 ? om[1] : 0ull;
+        const unsigned long long live0 = om[0], live1 = NWD > 1 ? om[1] : 0ull;
+        // element q sits dq = in_index(t, q) - t rows below the thread's first row
+        // for every thread: one per-thread vector offset, dq*row_step as the scalar
+        // offset
+        const unsigned vbase = col_off + (unsigned)t * row_step;
 #pragma unroll
         for (int q = 0; q < P::E; q++) {
-            const int row = P::in_index(t, q);
-            const int tv = row / kTile;
-            const bool live = active && (((tv < 64 ? m0 : m1) >> (tv & 63)) & 1ull);
-            v[q] = live ? in[base + (long long)row * rs] : T2{0, 0};
+            const int dq = P::in_index(0, q);
+            const int tv = (t + dq) / kTile;
+            const bool live = active && ((((tv < 64) ? live0 : live1) >> (tv & 63)) & 1ull);
+            const unsigned voff = live ? vbase : 0x80000000u;
+            const int soff = (int)((unsigned)dq * row_step);
+            if constexpr (sizeof(T2) == 16)
+                v[q] = __builtin_bit_cast(T2,
+                                          __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0));
+            else
+                v[q] = __builtin_bit_cast(T2,
+                                          __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0));
         }
     } else {
 #pragma unroll
@@ -412,10 +432,9 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
     constexpr int ROUNDS = R1 < W ? R1 : W;
     const int strips = (nx + W - 1) / W;
     const long long nstrips = (long long)strips * nw;
-    // the occupancy-checking variant exists for FP64 only (FP32 strips would spill)
-    constexpr bool kOcc = sizeof(Real) == 8;
-    if (occ && !kOcc) return hipErrorInvalidValue;
-    auto k = sign > 0 ? (occ ? k_fft_cols<Real, N, E, W, ROUNDS, 1, kOcc>
+    // the occupancy-reading variant addresses a plane with 32-bit offsets
+    if (occ && (long long)N * nx * (long long)sizeof(T2) >= (1LL << 31)) return hipErrorInvalidValue;
+    auto k = sign > 0 ? (occ ? k_fft_cols<Real, N, E, W, ROUNDS, 1, true>
                              : k_fft_cols<Real, N, E, W, ROUNDS, 1, false>)
                       : k_fft_cols<Real, N, E, W, ROUNDS, -1, false>;
     // strip-major planes: whole strips of a power-of-two width that W divides
