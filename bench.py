@@ -115,6 +115,9 @@ def bench_stream(dev, n_base=16 * 1024 * 1024, nchan=16, nx=8192, nw=128, suppor
     torch.cuda.synchronize()
     ms_img = (time.perf_counter() - t0) * 1e3
     print("bench: stream fp32c image %.1f ms" % ms_img, file=sys.stderr, flush=True)
+    img_stages = profile_stages(
+        lambda: imager.image(*vis, *args, precision="fp32c", cube=cube, image_real=re32,
+                             image_imag=im32), 1)
     # major cycles on the resident samples, reusing the FP32C cube as scratch
     op = ImagingOperator(s["uu"], s["vv"], s["ww"], None, nw, support, nx, nx, res, obs.wmin,
                          obs.wmax, precision="fp32c", cube=cube)
@@ -150,6 +153,7 @@ def bench_stream(dev, n_base=16 * 1024 * 1024, nchan=16, nx=8192, nw=128, suppor
                    "samples": n, "grid": [nx, nx], "w_planes": nw, "support": support},
         "stream_h2d_GBs": nbytes / secs / 1e9, "stream_ms": secs * 1e3,
         "image_ms": ms_img, "image_Mvis_s": n / (ms_img * 1e-3) / 1e6,
+        "image_stages_ms": {k: round(v[0], 3) for k, v in img_stages.items()},
         "image_algorithmic_GBs": (36 * n + 4 * cube_bytes + 16 * nx * nx) / (ms_img * 1e-3) / 1e9,
         "major_cycle_ms": ms_cyc, "cycles": cycles, "chi2": h,
         "image_rel_rms_err_vs_fp64": rms, "image_rel_max_err_vs_fp64": mx,

@@ -624,10 +624,25 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
             return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
                                    1.0, cg.st, occ, sm_log);
         }));
-        HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
-            return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp, scale,
-                                     image_real, image_imag, accumulate, cg.st, sm_log);
-        }));
+        if (grid_size_x >= 8192 && sm_log < 0) {
+            // rows of 8192 run the fused pass at 1024 threads a workgroup, where its
+            // per-pixel state does not fit the registers: the row transform (in
+            // place) and the phase/plane sum run as two passes instead
+            HIPCHK(stage(c, kStRows, cg.st, [&] {
+                return fft_rows<float>(1, cube, cube, twx, grid_size_x,
+                                       (long long)num_w_planes * grid_size_y, scale, cg.st);
+            }));
+            HIPCHK(stage(c, kStPhase, cg.st, [&] {
+                return phase_sum(LOFAR_FP32C, cube, image_real, image_imag, grid_size_x,
+                                 grid_size_y, num_w_planes, grid_size_x, grid_size_y, resolution,
+                                 wmin, wmax, accumulate, cg.st);
+            }));
+        } else {
+            HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
+                return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
+                                         scale, image_real, image_imag, accumulate, cg.st, sm_log);
+            }));
+        }
     }
     return LOFAR_OK;
 }

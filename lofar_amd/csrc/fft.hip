@@ -92,6 +92,15 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
         const int ntu = (nx + kTile - 1) / kTile;
         const unsigned long long* om = occ + (plane * ntu + u0 / kTile) * (NWD > 0 ? NWD : 1);
         const unsigned long long live0 = om[0], live1 = NWD > 1 ? om[1] : 0ull;
+        if ((live0 | live1) == 0ull) {
+            // no samples anywhere in this strip's tile column: its transform is
+            // zero, stored without loads or arithmetic
+            if (active) {
+#pragma unroll
+                for (int e = 0; e < P::E; e++) out[base + (long long)P::out_index(t, e) * rs] = T2{0, 0};
+            }
+            return;
+        }
         // element q sits dq = in_index(t, q) - t rows below the thread's first row
         // for every thread: one per-thread vector offset, dq*row_step as the scalar
         // offset

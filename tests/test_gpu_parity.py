@@ -353,6 +353,25 @@ def test_image_ignores_stale_scratch_cube(cuda, precision, tol):
         assert rel(H(r) + 1j * H(i), want) < tol
 
 
+def test_image_8192_split_row_pass(cuda):
+    """8192-wide FP32C grids run the row transform and the phase/plane sum as two passes
+    (the fused row pass does not fit its 1024-thread workgroups): the image agrees with the
+    FP64 mode (fused path) on the same samples, and accumulate mode adds to the image."""
+    from lofar_amd import imager
+    nx, nw = 8192, 3
+    d = synth(200000, nx, nw, seed=8192)
+    args = (nw, 7, nx, nx, d["resolution"], d["wmin"], d["wmax"])
+    r64, i64 = imager.image(*dev_inputs(d, cuda), *args, precision="fp64")
+    want = H(r64) + 1j * H(i64)
+    r32, i32 = imager.image(*dev_inputs(d, cuda), *args, precision="fp32c")
+    e = rel(H(r32) + 1j * H(i32), want)
+    print("8192 fp32c vs fp64 %.3e" % e)
+    assert e < 1e-6
+    imager.image(*dev_inputs(d, cuda), *args, precision="fp32c", image_real=r32, image_imag=i32,
+                 accumulate=True)
+    assert rel(H(r32) + 1j * H(i32), 2 * want) < 1e-6
+
+
 _LAYOUT_SCRIPT = r"""
 import sys, numpy as np, torch
 sys.path.insert(0, sys.argv[1])
