@@ -12,6 +12,7 @@ the kernels run on) and the reference CPU path timed on this host.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -279,10 +280,30 @@ def main():
 
         msA = time_steps(stepA, a.steps, a.warmup)
         profA = profile_stages(stepA, 3)
+        # dot test <y, A x> = <A^H y, x> on this workload: A x = the FP32C dirty image of the
+        # 16M visibilities, A^H y = the FP32C degridding of y = A x (normalised) + noise
+        ar, ai = imager.image(*vis, nw, support, nx, ny, d["resolution"], d["wmin"], d["wmax"],
+                              precision="fp32c", cube=cubeA)
+        rms = torch.sqrt(torch.mean(ar * ar + ai * ai))
+        yr = ar / rms + torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+        yi = ai / rms + torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+        imager.degrid(yr, yi, d["uu"], d["vv"], d["ww"], d["weight"], nw, support,
+                      d["resolution"], d["wmin"], d["wmax"], precision="fp32c", cube=cubeA,
+                      model_real=mr, model_imag=mi)
+        xr, xi = d["vis_real"].double(), d["vis_img"].double()
+        lhs_re = float((yr * ar + yi * ai).sum())
+        lhs_im = float((yr * ai - yi * ar).sum())
+        hr, hi = mr.double(), mi.double()
+        rhs_re = float((hr * xr + hi * xi).sum())
+        rhs_im = float((hr * xi - hi * xr).sum())
+        dot_rel = (math.hypot(lhs_re - rhs_re, lhs_im - rhs_im) / math.hypot(lhs_re, lhs_im))
+        del ar, ai, yr, yi, xr, xi, hr, hi
         extra["adjoint"] = {"workload": "degrid 4096x4096 image to 16M samples, 32 w-planes, FP32C",
                             "value": n2 / (msA * 1e-3) / 1e6, "unit": "Mvis/s", "ms_per_step": msA,
+                            "dot_test_rel_err": dot_rel,
                             "stages_ms": {k: round(v[0], 4) for k, v in profA.items()}}
-        print("bench: adjoint %.3f ms/step" % msA, file=sys.stderr, flush=True)
+        print("bench: adjoint %.3f ms/step, dot test %.2e" % (msA, dot_rel), file=sys.stderr,
+              flush=True)
         del cubeA, mr, mi
         torch.cuda.empty_cache()
 

@@ -443,6 +443,17 @@ int sm_log_cfg(int precision)
     return precision == LOFAR_FP64 ? 6 : -1;
 }
 
+// 8192-wide rows: split row passes (LOFAR_ROWS_SPLIT=0 keeps the fused ones, for
+// measurement)
+int rows_split()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("LOFAR_ROWS_SPLIT");
+        v = e ? atoi(e) != 0 : 1;
+    }
+    return v;
+}
 }  // namespace
 }  // namespace lofar
 
@@ -624,7 +635,7 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
             return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
                                    1.0, cg.st, occ, sm_log);
         }));
-        if (grid_size_x >= 8192 && sm_log < 0) {
+        if (grid_size_x >= 8192 && sm_log < 0 && rows_split()) {
             // rows of 8192 run the fused pass at 1024 threads a workgroup, where its
             // per-pixel state does not fit the registers: the row transform (in
             // place) and the phase/plane sum run as two passes instead
@@ -709,7 +720,29 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
         const PhaseParams pp =
             phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
         const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
-        if (precision == LOFAR_FP64) {
+        if (grid_size_x >= 8192 && rows_split()) {
+            // rows of 8192: the fused adjoint row pass does not fit its 1024-thread
+            // workgroups' registers, so the conjugate screen (every plane written) and
+            // the forward row transform (in place) run as two passes
+            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
+                const hipError_t e =
+                    phase_adjoint(precision, image_real, image_imag, cube, grid_size_x,
+                                  grid_size_y, num_w_planes, grid_size_x, grid_size_y,
+                                  resolution, wmin, wmax, cg.st);
+                if (e != hipSuccess) return e;
+                const long long lines = (long long)num_w_planes * grid_size_y;
+                return precision == LOFAR_FP64
+                           ? fft_rows<double>(-1, cube, cube, twx, grid_size_x, lines, 1.0, cg.st)
+                           : fft_rows<float>(-1, cube, cube, twx, grid_size_x, lines, 1.0, cg.st);
+            }));
+            HIPCHK(stage(c, kStCols, cg.st, [&] {
+                return precision == LOFAR_FP64
+                           ? fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                              num_w_planes, scale, cg.st)
+                           : fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                             num_w_planes, scale, cg.st);
+            }));
+        } else if (precision == LOFAR_FP64) {
             HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
                 return rows_phase_adj<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
                                               pp, 1.0, image_real, image_imag, cg.st);

@@ -700,7 +700,9 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
         const int k0 = (gv > g.K - 1) ? gv - g.K : 0;
         const int k1 = (gv < g.ny - g.K) ? gv + g.K : g.ny - 1;
         const int na = j1 - j0, nb = k1 - k0;
-        Real sr = 0, si = 0;
+        // sums in double in both modes: the window's terms cancel heavily for
+        // noise-like cubes, and float sums would cost ~1e-6 of the result
+        double sr = 0, si = 0;
         if (na >= 0 && nb >= 0) {
             Real eu[WIN];
 #pragma unroll
@@ -721,18 +723,18 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
                         f = norm * exp(-(dv * dv) * 0.5);
                     else
                         f = (float)norm * expf(-(float)(dv * dv) * 0.5f);
-                    Real rr = 0, ri = 0;
+                    double rr = 0, ri = 0;
                     const T2* row = base + kq * RS;
 #pragma unroll
                     for (int q = 0; q < WIN; q++) {
                         if (q <= na) {
                             const T2 x = row[q];
-                            rr += eu[q] * x.x;
-                            ri += eu[q] * x.y;
+                            rr += (double)eu[q] * (double)x.x;
+                            ri += (double)eu[q] * (double)x.y;
                         }
                     }
-                    sr += f * rr;
-                    si += f * ri;
+                    sr += (double)f * rr;
+                    si += (double)f * ri;
                 }
             }
         }

@@ -454,6 +454,31 @@ def test_degrid_matches_oracle_adjoint(cuda, precision, tol):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_adjoint_dot_test_8192(cuda, precision, tol):
+    """8192-wide grids run the adjoint's conjugate screen and row transform as two passes:
+    the dot-product identity against the imaging path still holds."""
+    from lofar_amd import imager
+    nx, nw = 8192, 2
+    d = synth(50000, nx, nw, seed=19)
+    x = d["vis_real"].astype(np.float64) + 1j * d["vis_img"].astype(np.float64)
+    r, i = imager.image(*dev_inputs(d, cuda), nw, 7, nx, nx, d["resolution"], d["wmin"],
+                        d["wmax"], precision=precision)
+    ax = H(r) + 1j * H(i)
+    rng = np.random.default_rng(20)
+    # y = Ax (normalised) plus noise: <y, Ax> is then not a zero-mean sum whose
+    # occasional small values would inflate the relative error of a correct adjoint
+    y = ax / np.sqrt(np.mean(np.abs(ax) ** 2)) + (rng.standard_normal((nx, nx)) +
+                                                   1j * rng.standard_normal((nx, nx)))
+    mr, mi = imager.degrid(T(y.real.copy(), cuda), T(y.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), nw, 7,
+                           d["resolution"], d["wmin"], d["wmax"], precision=precision)
+    ahy = H(mr).astype(np.float64) + 1j * H(mi).astype(np.float64)
+    lhs, rhs = np.vdot(y, ax), np.vdot(ahy, x)
+    print("dot test 8192 %s: %.3e" % (precision, abs(lhs - rhs) / abs(lhs)))
+    assert abs(lhs - rhs) < tol * abs(lhs)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
 def test_adjoint_dot_test(cuda, precision, tol):
     import torch
     from lofar_amd import imager
