@@ -16,6 +16,34 @@ namespace lofar {
 template <typename Real>
 using C2 = typename cplx<Real>::type;
 
+// Raw buffer load of one complex value at byte offset voff + soff (soff
+// wave-uniform, held in a scalar register).
+template <typename T2>
+__device__ __forceinline__ T2 load_soff(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(T2) == 16)
+        return __builtin_bit_cast(T2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(T2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Raw buffer store of one complex value at byte offset voff + soff.  The whole
+// offset goes in the vector register and the scalar offset field is 0: with a
+// scalar register there, the compiler does not keep the next VALU instruction
+// from overwriting the data registers of a 128-bit store before the store has
+// read them, and complex-double stores wrote the following element's value.
+template <typename T2>
+__device__ __forceinline__ void store_at(T2 x, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                           unsigned soff)
+{
+    using U4 = unsigned __attribute__((ext_vector_type(4)));
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    if constexpr (sizeof(T2) == 16)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, x), r, voff + soff, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, x), r, voff + soff, 0, 0);
+}
+
 // ---------------------------------------------------------------------------
 // standalone passes
 
@@ -59,6 +87,10 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
 {
     using T2 = C2<Real>;
     using P = Plan<N, E>;
+    // complex doubles are stored through a buffer resource (one vector offset per
+    // thread, no 64-bit address arithmetic per element), complex floats with
+    // global stores (measured as fast, with fewer instructions around them)
+    constexpr bool kBufStore = sizeof(T2) >= 16;
     constexpr int WR = W / ROUNDS;  // columns sharing the LDS buffer in one round
     __shared__ T2 lds[P::LDS * WR];
     const long long s = xcd_remap(blockIdx.x, gridDim.x);
@@ -70,22 +102,26 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
     // row-major planes: rows nx apart; strip-major planes ([nx/L][N][L], L = 2^sm_log,
     // a multiple of W): rows L apart inside the strip holding these columns
     const long long rs = sm_log >= 0 ? (1LL << sm_log) : nx;
+    const long long plane0 = plane * (long long)N * nx;
     const long long base =
-        plane * (long long)N * nx +
+        plane0 +
         (sm_log >= 0 ? ((long long)(u0 >> sm_log) * N << sm_log) + (u0 & ((1 << sm_log) - 1)) + c
                      : (active ? u0 + c : 0));
+    // Element q sits dq = in_index(t, q) - t rows below the thread's first row, and
+    // output e at out_index(t, e) - t, the same for every thread: addresses are one
+    // per-thread byte offset into the plane plus a scalar offset per element,
+    // through buffer resources over the plane (planes are below 2 GiB: at most
+    // 8192 x 8192 complex doubles).
+    const int plane_bytes = (int)((long long)N * nx * (long long)sizeof(T2));
+    const unsigned row_step = (unsigned)(rs * (long long)sizeof(T2));
+    const unsigned vbase =
+        (unsigned)((base - plane0) * (long long)sizeof(T2)) + (unsigned)t * row_step;
     T2 v[P::E];
     if constexpr (OCC) {
-        // Loads through a buffer resource over this plane: 32-bit byte offsets, and
-        // an offset past the plane (rows of tiles without samples, columns past the
-        // grid) returns zeros without a memory access.  Planes are below 2 GiB
-        // (launch_cols_cfg checks).
-        const long long plane0 = plane * (long long)N * nx;
+        // An offset past the plane (rows of tiles without samples, columns past the
+        // grid) reads zeros without a memory access.
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(in + plane0), (short)0, (int)((long long)N * nx * (long long)sizeof(T2)),
-            0x00020000);  // raw, range-checked
-        const unsigned col_off = (unsigned)((base - plane0) * (long long)sizeof(T2));
-        const unsigned row_step = (unsigned)(rs * (long long)sizeof(T2));
+            (void*)(in + plane0), (short)0, plane_bytes, 0x00020000);  // raw, range-checked
         // bit tv of word tv/64: tile row tv of this strip's tile column holds samples
         constexpr int NWD = (N / kTile + 63) / 64;
         static_assert(NWD <= 2, "occupancy reads cover at most 128 tile rows");
@@ -96,28 +132,23 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
             // no samples anywhere in this strip's tile column: its transform is
             // zero, stored without loads or arithmetic
             if (active) {
+                const __amdgpu_buffer_rsrc_t wsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void*)(out + plane0), (short)0, plane_bytes, 0x00020000);
 #pragma unroll
-                for (int e = 0; e < P::E; e++) out[base + (long long)P::out_index(t, e) * rs] = T2{0, 0};
+                for (int e = 0; e < P::E; e++)
+                    store_at(T2{0, 0}, wsrc, vbase,
+                               (unsigned)(P::out_index(0, e)) * row_step);
             }
             return;
         }
-        // element q sits dq = in_index(t, q) - t rows below the thread's first row
-        // for every thread: one per-thread vector offset, dq*row_step as the scalar
-        // offset
-        const unsigned vbase = col_off + (unsigned)t * row_step;
 #pragma unroll
         for (int q = 0; q < P::E; q++) {
             const int dq = P::in_index(0, q);
             const int tv = (t + dq) / kTile;
             const bool live = active && ((((tv < 64) ? live0 : live1) >> (tv & 63)) & 1ull);
+            // offsets of live rows are below 2^31; dead rows read past the plane
             const unsigned voff = live ? vbase : 0x80000000u;
-            const int soff = (int)((unsigned)dq * row_step);
-            if constexpr (sizeof(T2) == 16)
-                v[q] = __builtin_bit_cast(T2,
-                                          __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0));
-            else
-                v[q] = __builtin_bit_cast(T2,
-                                          __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0));
+            v[q] = load_soff<T2>(rsrc, voff, (unsigned)dq * row_step);
         }
     } else {
 #pragma unroll
@@ -126,12 +157,17 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
     }
     stages<T2, N, E, SIGN, 1, ROUNDS>(v, lds + c % WR, WR, t, tw, c / WR);
     if (active) {
+        const __amdgpu_buffer_rsrc_t wsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(out + plane0), (short)0, plane_bytes, 0x00020000);
 #pragma unroll
         for (int e = 0; e < P::E; e++) {
             T2 x = v[e];
             x.x *= scale;
             x.y *= scale;
-            out[base + (long long)P::out_index(t, e) * rs] = x;
+            if constexpr (kBufStore)
+                store_at(x, wsrc, vbase, (unsigned)(P::out_index(0, e)) * row_step);
+            else
+                out[base + (long long)P::out_index(t, e) * rs] = x;
         }
     }
 }

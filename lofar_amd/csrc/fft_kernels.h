@@ -32,6 +32,10 @@ struct Plan {
     // exchange (butterfly b writes b*R0..b*R0+R0-1) does not put every lane of
     // a wave on the same banks
     __device__ static __forceinline__ int pad(int i) { return i + (i >> ilog2(R0)); }
+    // pad(x + c) = pad(x) + padc(c) when c is a multiple of R0, or when x is and
+    // c < R0: the exchanges address their elements as one runtime slot plus
+    // compile-time offsets (immediate offsets of the LDS instructions)
+    static constexpr int padc(int c) { return c + (c >> ilog2(R0)); }
     static constexpr int LDS = N + N / R0;  // slots per line
 };
 
@@ -223,14 +227,26 @@ __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* 
 #pragma unroll
         for (int rr = 0; rr < ROUNDS; rr++) {
             __syncthreads();
+            static_assert(NS == 1 ? R == P::R0 : NS % P::R0 == 0, "linear write offsets");
+            static_assert((N / R2) % P::R0 == 0, "linear read offsets");
+            // complex doubles: one runtime slot per butterfly plus compile-time
+            // offsets (fewer integer instructions, measured faster); complex floats
+            // keep a slot computed per element (the linear form showed no gain there)
+            constexpr bool LIN = sizeof(T2) >= 16;
             if (rr == my_round) {
 #pragma unroll
                 for (int m = 0; m < M; m++) {
                     const int b = t + m * TPL;
                     const int k = b & (NS - 1);
                     const int base = (b - k) * R + k;
+                    T2* row = lds + P::pad(base) * ls;
 #pragma unroll
-                    for (int q = 0; q < R; q++) lds[P::pad(base + q * NS) * ls] = v[m * R + q];
+                    for (int q = 0; q < R; q++) {
+                        if constexpr (LIN)
+                            row[P::padc(q * NS) * ls] = v[m * R + q];
+                        else
+                            lds[P::pad(base + q * NS) * ls] = v[m * R + q];
+                    }
                 }
             }
             __syncthreads();
@@ -238,9 +254,14 @@ __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* 
 #pragma unroll
                 for (int m = 0; m < M2; m++) {
                     const int b = t + m * TPL;
+                    const T2* row = lds + P::pad(b) * ls;
 #pragma unroll
-                    for (int q = 0; q < R2; q++)
-                        v[m * R2 + q] = lds[P::pad(b + q * (N / R2)) * ls];
+                    for (int q = 0; q < R2; q++) {
+                        if constexpr (LIN)
+                            v[m * R2 + q] = row[P::padc(q * (N / R2)) * ls];
+                        else
+                            v[m * R2 + q] = lds[P::pad(b + q * (N / R2)) * ls];
+                    }
                 }
             }
         }

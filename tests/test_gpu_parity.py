@@ -212,7 +212,7 @@ def test_wstack_dropin_error_leaves_grid(cuda):
 # plane transform
 
 @pytest.mark.parametrize("nx,ny,nw", [(2, 2, 3), (4, 16, 2), (16, 16, 2), (32, 8, 1), (64, 64, 3),
-                                      (128, 512, 1), (256, 256, 2), (1024, 1024, 1),
+                                      (128, 512, 1), (256, 256, 2), (1024, 1024, 1), (1024, 1024, 4),
                                       (2048, 64, 1), (64, 2048, 1), (4096, 4096, 1)])
 @pytest.mark.parametrize("sign", [1, -1])
 def test_fft_matches_numpy(cuda, nx, ny, nw, sign):
