@@ -108,10 +108,9 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
         (sm_log >= 0 ? ((long long)(u0 >> sm_log) * N << sm_log) + (u0 & ((1 << sm_log) - 1)) + c
                      : (active ? u0 + c : 0));
     // Element q sits dq = in_index(t, q) - t rows below the thread's first row, and
-    // output e at out_index(t, e) - t, the same for every thread: addresses are one
-    // per-thread byte offset into the plane plus a scalar offset per element,
-    // through buffer resources over the plane (planes are below 2 GiB: at most
-    // 8192 x 8192 complex doubles).
+    // output e at out_index(t, e) - t, the same for every thread: buffer accesses
+    // are one per-thread byte offset into the plane plus a per-element offset
+    // (planes are below 2 GiB, launch_cols_cfg checks: 32-bit offsets).
     const int plane_bytes = (int)((long long)N * nx * (long long)sizeof(T2));
     const unsigned row_step = (unsigned)(rs * (long long)sizeof(T2));
     const unsigned vbase =
@@ -477,8 +476,8 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
     constexpr int ROUNDS = R1 < W ? R1 : W;
     const int strips = (nx + W - 1) / W;
     const long long nstrips = (long long)strips * nw;
-    // the occupancy-reading variant addresses a plane with 32-bit offsets
-    if (occ && (long long)N * nx * (long long)sizeof(T2) >= (1LL << 31)) return hipErrorInvalidValue;
+    // the kernel addresses a plane with 32-bit byte offsets
+    if ((long long)N * nx * (long long)sizeof(T2) >= (1LL << 31)) return hipErrorInvalidValue;
     auto k = sign > 0 ? (occ ? k_fft_cols<Real, N, E, W, ROUNDS, 1, true>
                              : k_fft_cols<Real, N, E, W, ROUNDS, 1, false>)
                       : k_fft_cols<Real, N, E, W, ROUNDS, -1, false>;
