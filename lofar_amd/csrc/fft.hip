@@ -115,6 +115,12 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
     const unsigned row_step = (unsigned)(rs * (long long)sizeof(T2));
     const unsigned vbase =
         (unsigned)((base - plane0) * (long long)sizeof(T2)) + (unsigned)t * row_step;
+    // complex doubles: the stages' twiddles as powers of roots loaded here, ahead
+    // of the column data
+    constexpr bool kRoots = sizeof(T2) >= 16 && W * P::TPL <= 512;  // not at the 128-VGPR cap
+    constexpr int NRT = kRoots ? root_count<N, E>(1) : 0;
+    T2 rt[NRT > 0 ? NRT : 1];
+    if constexpr (NRT > 0) load_roots<T2, N, E, 1>(rt, t, tw);
     T2 v[P::E];
     if constexpr (OCC) {
         // An offset past the plane (rows of tiles without samples, columns past the
@@ -154,7 +160,7 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
         for (int q = 0; q < P::E; q++)
             v[q] = active ? in[base + (long long)P::in_index(t, q) * rs] : T2{0, 0};
     }
-    stages<T2, N, E, SIGN, 1, ROUNDS>(v, lds + c % WR, WR, t, tw, c / WR);
+    stages<T2, N, E, SIGN, 1, ROUNDS, (NRT > 0)>(v, lds + c % WR, WR, t, tw, c / WR, rt);
     if (active) {
         const __amdgpu_buffer_rsrc_t wsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(out + plane0), (short)0, plane_bytes, 0x00020000);

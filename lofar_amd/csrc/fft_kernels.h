@@ -200,9 +200,13 @@ __device__ __forceinline__ void dft(T2* v)
 // the lines of a workgroup take turns on a shared LDS buffer: a thread
 // exchanges in round my_round only (all threads pass every barrier).
 // Twiddles: tw[m] = exp(+2*pi*i*m/N), conjugated for SIGN < 0.
-template <typename T2, int N, int EREQ, int SIGN, int NS, int ROUNDS = 1>
+// RT: the twiddles of butterfly m are the powers w^q of one root per butterfly,
+// rt[m] = tw[k*N/(NS*R)] (load_roots), taken from registers loaded when the
+// kernel starts instead of R-1 table loads at the stage (whose latency a stage
+// otherwise waits out); q-1 complex products per power, a few ulp in FP64.
+template <typename T2, int N, int EREQ, int SIGN, int NS, int ROUNDS = 1, bool RT = false>
 __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* __restrict__ tw,
-                                       int my_round = 0)
+                                       int my_round = 0, const T2* rt = nullptr)
 {
     using P = Plan<N, EREQ>;
     constexpr int E = P::E;
@@ -213,10 +217,20 @@ __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* 
     for (int m = 0; m < M; m++) {
         const int b = t + m * TPL;
         if constexpr (NS > 1) {
-            const int k = b & (NS - 1);
+            if constexpr (RT) {
+                const T2 r = rt[m];
+                T2 w = r;
 #pragma unroll
-            for (int q = 1; q < R; q++)
-                v[m * R + q] = c_mul_tw<SIGN>(v[m * R + q], tw[q * k * (N / (NS * R))]);
+                for (int q = 1; q < R; q++) {
+                    v[m * R + q] = c_mul_tw<SIGN>(v[m * R + q], w);
+                    if (q + 1 < R) w = c_mul_tw<1>(w, r);
+                }
+            } else {
+                const int k = b & (NS - 1);
+#pragma unroll
+                for (int q = 1; q < R; q++)
+                    v[m * R + q] = c_mul_tw<SIGN>(v[m * R + q], tw[q * k * (N / (NS * R))]);
+            }
         }
         dft<R, SIGN>(&v[m * R]);
     }
@@ -265,8 +279,41 @@ __device__ __forceinline__ void stages(T2* v, T2* lds, int ls, int t, const T2* 
                 }
             }
         }
-        stages<T2, N, EREQ, SIGN, NS2, ROUNDS>(v, lds, ls, t, tw, my_round);
+        stages<T2, N, EREQ, SIGN, NS2, ROUNDS, RT>(v, lds, ls, t, tw, my_round,
+                                                  RT ? rt + (NS > 1 ? M : 0) : rt);
     }
+}
+
+// Number of twiddle roots stages<..., RT> uses from span NS on (one per butterfly
+// of every stage after the first).
+template <int N, int EREQ>
+constexpr int root_count(int ns)
+{
+    using P = Plan<N, EREQ>;
+    int total = 0;
+    while (ns < N) {
+        const int r = (N / ns >= P::RX) ? P::RX : N / ns;
+        if (ns > 1) total += P::E / r;
+        ns *= r;
+    }
+    return total;
+}
+
+// Loads the roots stages<..., RT> consumes, in its order.
+template <typename T2, int N, int EREQ, int NS>
+__device__ __forceinline__ void load_roots(T2* rt, int t, const T2* __restrict__ tw)
+{
+    using P = Plan<N, EREQ>;
+    constexpr int R = (N / NS >= P::RX) ? P::RX : N / NS;
+    constexpr int M = P::E / R;
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int k = (t + m * P::TPL) & (NS - 1);
+            rt[m] = tw[k * (N / (NS * R))];
+        }
+    }
+    if constexpr (NS * R < N) load_roots<T2, N, EREQ, NS * R>(rt + (NS > 1 ? M : 0), t, tw);
 }
 
 // Number of twiddle factors one thread applies from span NS to the last stage.
