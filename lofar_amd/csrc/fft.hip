@@ -319,6 +319,12 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
 #pragma unroll
     for (int e = 0; e < P::E; e++) {
         ph[e].init(pp, pixel_fac(pp, P::out_index(t, e), ycoord));
+        if constexpr (RECUR) {
+            // the transform's scale 1/(nx*ny) (a power of two: exact) rides in the
+            // recurrence's first factor
+            ph[e].z.x *= scale;
+            ph[e].z.y *= scale;
+        }
         acc[e] = double2{0.0, 0.0};
     }
     T2 twr[NTW > 0 ? NTW : 1];
@@ -358,10 +364,18 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_rows_phase(const C2<Real>
         const double c2w = 2.0 * LOFAR_REF_PI * (pp.wmin + iw * pp.dw);
 #pragma unroll
         for (int e = 0; e < P::E; e++) {
-            const double2 pq = double2{(double)v[e].x * scale, (double)v[e].y * scale};
-            const double2 term = cmul(pq, ph[e].next(pp, c2w));
-            acc[e].x += term.x;
-            acc[e].y += term.y;
+            if constexpr (RECUR) {
+                // acc += v*z as four fused multiply-adds
+                const double2 z = ph[e].next(pp, c2w);
+                const double vx = (double)v[e].x, vy = (double)v[e].y;
+                acc[e].x = fma(vx, z.x, fma(-vy, z.y, acc[e].x));
+                acc[e].y = fma(vx, z.y, fma(vy, z.x, acc[e].y));
+            } else {
+                const double2 pq = double2{(double)v[e].x * scale, (double)v[e].y * scale};
+                const double2 term = cmul(pq, ph[e].next(pp, c2w));
+                acc[e].x += term.x;
+                acc[e].y += term.y;
+            }
         }
     }
 #pragma unroll
