@@ -14,8 +14,10 @@ import argparse
 import json
 import math
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -98,17 +100,139 @@ def bench_dropin(d, n, nx, ny, nw, support):
                 **{k: round(v, 1) for k, v in t.items()})
 
 
-def cpu_baseline(n, nx, nw):
+FIELDS = ("uu", "vv", "ww", "vis_real", "vis_img", "weight")
+
+
+def save_workload(work, name, d, nx, nw, support, img_re, img_im):
+    """Samples and the GPU image of one workload, for the cpu_baseline leg."""
+    import numpy as np
+    np.savez(os.path.join(work, name + ".npz"), **{k: d[k].cpu().numpy() for k in FIELDS},
+             image=img_re.cpu().numpy() + 1j * img_im.cpu().numpy(), nx=nx, ny=nx, nw=nw,
+             support=support, resolution=d["resolution"], wmin=d["wmin"], wmax=d["wmax"])
+
+
+def cpu_baseline(work, names):
+    """The reference CPU path on this host (oracle/cpu_baseline.py, a child process): its
+    timing on the first workload, and every workload's GPU image error against it."""
     env = dict(os.environ)
-    env.setdefault("MKL_NUM_THREADS", "1")
-    env.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
+    env["MKL_NUM_THREADS"] = "1"  # the FFT leg is a single-threaded FFTW_ESTIMATE plan
     try:
         out = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"),
-                              "--n", str(n), "--nx", str(nx), "--nw", str(nw)],
-                             capture_output=True, text=True, env=env, timeout=300)
+                              "--work", work, "--names", ",".join(names)],
+                             capture_output=True, text=True, env=env, timeout=900)
+        sys.stderr.write(out.stderr[-2000:])
         return json.loads(out.stdout.strip().splitlines()[-1])
     except Exception as e:  # reported, never fatal
         return {"error": repr(e)}
+
+
+# 64-bit LDS atomics per clock per CU that the LDS's 128 bytes/clock/CU could carry; MI355X:
+# 256 CUs at 2.4 GHz.  A bandwidth-derived ceiling: AMD publishes no LDS atomic rate.
+LDS_ATOMIC_PEAK_G = 16 * 256 * 2.4
+
+
+def lds_atomics(n, support, prof):
+    """Secondary bound of the gridder: 64-bit LDS atomic adds it needs (re and im of every
+    cell a sample's window covers, 2*S^2 per sample) per second of grid_tiles time."""
+    if "grid_tiles" not in prof:
+        return None
+    ms = prof["grid_tiles"][0] / max(prof["grid_tiles"][1], 1)
+    ops = 2 * support * support * n
+    g = ops / (ms * 1e-3) / 1e9
+    return {"kernel": "grid_tiles", "algorithmic_atomics": ops, "avg_launch_ms": ms,
+            "achieved": g, "peak": LDS_ATOMIC_PEAK_G, "unit": "G 64-bit LDS atomics/s",
+            "frac": g / LDS_ATOMIC_PEAK_G,
+            "peak_basis": "16 per clock per CU (128 B/clock/CU LDS), 256 CUs, 2.4 GHz"}
+
+
+TRAFFIC_SOURCE = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of "
+                  "this workload (scaled by the copy calibration), committed beside the kernel "
+                  "profiles; not measured in this run")
+
+
+def bench_adjoint(d, vis, cubeA, n2, nx, ny, nw, support, a):
+    """Degridding a 4096^2 image to the 16M samples (FP32C), and the dot test
+    <y, A x> = <A^H y, x> on that workload."""
+    import torch
+    from lofar_amd import imager
+    dev = cubeA.device
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    img_re = torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+    img_im = torch.zeros_like(img_re)
+    mr = torch.empty(n2, dtype=torch.float32, device=dev)
+    mi = torch.empty_like(mr)
+
+    def stepA():
+        imager.degrid(img_re, img_im, d["uu"], d["vv"], d["ww"], d["weight"], nw, support,
+                      d["resolution"], d["wmin"], d["wmax"], precision="fp32c", cube=cubeA,
+                      model_real=mr, model_imag=mi)
+
+    msA = time_steps(stepA, a.steps, a.warmup)
+    profA = profile_stages(stepA, 3)
+    # A x = the FP32C dirty image of the 16M visibilities, A^H y = the FP32C degridding of
+    # y = A x (normalised) + noise
+    ar, ai = imager.image(*vis, nw, support, nx, ny, d["resolution"], d["wmin"], d["wmax"],
+                          precision="fp32c", cube=cubeA)
+    rms = torch.sqrt(torch.mean(ar * ar + ai * ai))
+    yr = ar / rms + torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+    yi = ai / rms + torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+    imager.degrid(yr, yi, d["uu"], d["vv"], d["ww"], d["weight"], nw, support,
+                  d["resolution"], d["wmin"], d["wmax"], precision="fp32c", cube=cubeA,
+                  model_real=mr, model_imag=mi)
+    xr, xi = d["vis_real"].double(), d["vis_img"].double()
+    lhs_re = float((yr * ar + yi * ai).sum())
+    lhs_im = float((yr * ai - yi * ar).sum())
+    hr, hi = mr.double(), mi.double()
+    rhs_re = float((hr * xr + hi * xi).sum())
+    rhs_im = float((hr * xi - hi * xr).sum())
+    dot_rel = (math.hypot(lhs_re - rhs_re, lhs_im - rhs_im) / math.hypot(lhs_re, lhs_im))
+    print("bench: adjoint %.3f ms/step, dot test %.2e" % (msA, dot_rel), file=sys.stderr,
+          flush=True)
+    C = nw * nx * ny * 8
+    alg = 16 * nx * ny + 4 * C + 44 * n2
+    return {"workload": "degrid 4096x4096 image to 16M samples, 32 w-planes, FP32C",
+            "value": n2 / (msA * 1e-3) / 1e6, "unit": "Mvis/s", "ms_per_step": msA,
+            "algorithmic_GBs": alg / (msA * 1e-3) / 1e9, "dot_test_rel_err": dot_rel,
+            "stages_ms": {k: round(v[0], 4) for k, v in profA.items()}}
+
+
+def bench_chi2(d, n2, nx, ny, nw, support):
+    """chi^2 sky-model fit: 20 gradient-descent iterations through the autograd binding,
+    each one degrid (forward) + one imaging of the weighted residuals (backward)."""
+    import torch
+    from lofar_amd.autograd import ImagingOperator, chi2_descent, lipschitz
+    dev = d["uu"].device
+    g = torch.Generator(device=dev)
+    g.manual_seed(6)
+    op = ImagingOperator(d["uu"], d["vv"], d["ww"], None, nw, support, nx, ny,
+                         d["resolution"], d["wmin"], d["wmax"], precision="fp32c")
+    yy, xx = torch.meshgrid(torch.arange(ny, device=dev, dtype=torch.float64),
+                            torch.arange(nx, device=dev, dtype=torch.float64), indexing="ij")
+    sky = 1e-3 * torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
+    for (cy, cx, amp, sig) in ((1200.0, 900.0, 1.0, 3.0), (2900.0, 2500.0, 0.6, 5.0),
+                               (700.0, 3300.0, 1.5, 2.0)):
+        sky += amp * torch.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sig * sig))
+    del yy, xx
+    zero = torch.zeros_like(sky)
+    step = 0.5 / lipschitz(op, d["weight"], iters=4)
+    hist = []
+    chi2_descent(op, d["vis_real"], d["vis_img"], d["weight"], sky, zero, iters=1, step=step)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    chi2_descent(op, d["vis_real"], d["vis_img"], d["weight"], sky, zero, iters=20, step=step,
+                 history=hist)
+    torch.cuda.synchronize()
+    msG = (time.perf_counter() - t0) * 1e3 / 20
+    h = [float(v) for v in hist]
+    print("bench: chi2 fit %.3f ms/iteration, chi2 %.4e -> %.4e" % (msG, h[0], h[-1]),
+          file=sys.stderr, flush=True)
+    return {"workload": "20 gradient-descent iterations of the weighted chi^2 of a 4096x4096 "
+                        "sky model against 16M samples, 32 w-planes, FP32C, torch.autograd "
+                        "(forward degrid + backward imaging of residuals)",
+            "value": n2 / (msG * 1e-3) / 1e6, "unit": "Mvis/s per iteration",
+            "ms_per_iteration": msG, "chi2_first": h[0], "chi2_last": h[-1],
+            "monotone": all(b <= a * (1 + 1e-6) for a, b in zip(h, h[1:]))}
 
 
 def load_traffic(key):
@@ -192,7 +316,9 @@ def bench_stream(dev, n_base=16 * 1024 * 1024, nchan=16, nx=8192, nw=128, suppor
         "major_cycle_ms": ms_cyc, "cycles": cycles, "chi2": h,
         "image_rel_rms_err_vs_fp64": rms, "image_rel_max_err_vs_fp64": mx,
     }
-    print(json.dumps(line), flush=True)
+    del s, vis, re32, im32, r64, i64
+    torch.cuda.empty_cache()
+    return line
 
 
 def main():
@@ -201,7 +327,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline leg (and with it the image errors against the "
+                         "reference CPU path)")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the multi-channel streaming workload")
     ap.add_argument("--stream", action="store_true",
                     help="only the multi-channel streaming workload: 256M samples (16M "
                          "baseline samples x 16 channels, 120-168 MHz) streamed to the device "
@@ -223,7 +353,8 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if a.stream:
-        return bench_stream(dev)
+        print(json.dumps(bench_stream(dev)), flush=True)
+        return
     if a.calibrate_copy:
         src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -238,7 +369,7 @@ def main():
     n2 = 16 * 1024 * 1024
     print("bench: generating %d samples (%dx%d, %d planes)" % (n2, nx, ny, nw), file=sys.stderr, flush=True)
     d = make_obs(n2, nx, nw, seed=2024, backend="torch", device=dev)
-    vis = [d[k] for k in ("uu", "vv", "ww", "vis_real", "vis_img", "weight")]
+    vis = [d[k] for k in FIELDS]
     cube64 = torch.empty((nw, ny, nx), dtype=torch.complex128, device=dev)
     re64 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
     im64 = torch.empty_like(re64)
@@ -258,18 +389,25 @@ def main():
     roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": load_traffic("fp64_4096x4096x32_16M:" + dom),
+                "traffic_source": TRAFFIC_SOURCE,
                 "algorithmic_bytes": kb[dom], "avg_launch_ms": dom_ms,
                 "stages_ms": {k: round(v[0], 4) for k, v in prof.items()},
-                "end_to_end_GBs": kb["total"] / (ms64 * 1e-3) / 1e9}
+                "end_to_end_GBs": kb["total"] / (ms64 * 1e-3) / 1e9,
+                "gridder_lds_atomics": lds_atomics(n2, support, prof)}
+    work = tempfile.mkdtemp(prefix="lofar_bench_")
+    names = []
+    if not a.no_cpu:
+        save_workload(work, "fp64", d, nx, nw, support, re64, im64)
+        names.append("fp64")
 
     extra = {}
     if not a.no_extra:
-        # FP32C on 64M samples, same grid; error against the FP64 mode on the same input
+        # FP32C on 64M samples, same grid
         del cube64
         torch.cuda.empty_cache()
         n3 = 64 * 1024 * 1024
         d3 = make_obs(n3, nx, nw, seed=2025, backend="torch", device=dev)
-        vis3 = [d3[k] for k in ("uu", "vv", "ww", "vis_real", "vis_img", "weight")]
+        vis3 = [d3[k] for k in FIELDS]
         cube32 = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
         re32 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
         im32 = torch.empty_like(re32)
@@ -280,104 +418,48 @@ def main():
 
         ms32 = time_steps(step32, a.steps, a.warmup)
         prof32 = profile_stages(step32, 3)
-        del cube32
-        torch.cuda.empty_cache()
-        c64 = torch.empty((nw, ny, nx), dtype=torch.complex128, device=dev)
-        r64, i64 = imager.image(*vis3, nw, support, nx, ny, d3["resolution"], d3["wmin"],
-                                d3["wmax"], precision="fp64", cube=c64)
-        rms, mx = rel_err(re32, im32, r64, i64)
         mvis32 = n3 / (ms32 * 1e-3) / 1e6
         extra["fp32c"] = {"workload": "64M samples, 4096x4096, 32 w-planes, support 7, complex64 cube",
                           "value": mvis32, "unit": "Mvis/s", "ms_per_step": ms32,
                           "speedup_vs_fp64_mode": mvis32 / value,
-                          "image_rel_rms_err_vs_fp64": rms, "image_rel_max_err_vs_fp64": mx,
-                          "stages_ms": {k: round(v[0], 4) for k, v in prof32.items()}}
-        print("bench: fp32c %.3f ms/step %.1f Mvis/s err %.2e" % (ms32, mvis32, rms),
-              file=sys.stderr, flush=True)
-        del d3, vis3, c64
-        torch.cuda.empty_cache()
-
-        # adjoint: degridding a 4096^2 image to the 16M samples (FP32C)
-        cubeA = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
-        g = torch.Generator(device=dev)
-        g.manual_seed(5)
-        img_re = torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
-        img_im = torch.zeros_like(img_re)
-        mr = torch.empty(n2, dtype=torch.float32, device=dev)
-        mi = torch.empty_like(mr)
-
-        def stepA():
-            imager.degrid(img_re, img_im, d["uu"], d["vv"], d["ww"], d["weight"], nw, support,
-                          d["resolution"], d["wmin"], d["wmax"], precision="fp32c", cube=cubeA,
-                          model_real=mr, model_imag=mi)
-
-        msA = time_steps(stepA, a.steps, a.warmup)
-        profA = profile_stages(stepA, 3)
-        # dot test <y, A x> = <A^H y, x> on this workload: A x = the FP32C dirty image of the
-        # 16M visibilities, A^H y = the FP32C degridding of y = A x (normalised) + noise
-        ar, ai = imager.image(*vis, nw, support, nx, ny, d["resolution"], d["wmin"], d["wmax"],
-                              precision="fp32c", cube=cubeA)
-        rms = torch.sqrt(torch.mean(ar * ar + ai * ai))
-        yr = ar / rms + torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
-        yi = ai / rms + torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
-        imager.degrid(yr, yi, d["uu"], d["vv"], d["ww"], d["weight"], nw, support,
-                      d["resolution"], d["wmin"], d["wmax"], precision="fp32c", cube=cubeA,
-                      model_real=mr, model_imag=mi)
-        xr, xi = d["vis_real"].double(), d["vis_img"].double()
-        lhs_re = float((yr * ar + yi * ai).sum())
-        lhs_im = float((yr * ai - yi * ar).sum())
-        hr, hi = mr.double(), mi.double()
-        rhs_re = float((hr * xr + hi * xi).sum())
-        rhs_im = float((hr * xi - hi * xr).sum())
-        dot_rel = (math.hypot(lhs_re - rhs_re, lhs_im - rhs_im) / math.hypot(lhs_re, lhs_im))
-        del ar, ai, yr, yi, xr, xi, hr, hi
-        extra["adjoint"] = {"workload": "degrid 4096x4096 image to 16M samples, 32 w-planes, FP32C",
-                            "value": n2 / (msA * 1e-3) / 1e6, "unit": "Mvis/s", "ms_per_step": msA,
-                            "dot_test_rel_err": dot_rel,
-                            "stages_ms": {k: round(v[0], 4) for k, v in profA.items()}}
-        print("bench: adjoint %.3f ms/step, dot test %.2e" % (msA, dot_rel), file=sys.stderr,
+                          "stages_ms": {k: round(v[0], 4) for k, v in prof32.items()},
+                          "gridder_lds_atomics": lds_atomics(n3, support, prof32)}
+        print("bench: fp32c %.3f ms/step %.1f Mvis/s" % (ms32, mvis32), file=sys.stderr,
               flush=True)
-        del cubeA, mr, mi
+        if not a.no_cpu:
+            save_workload(work, "fp32c", d3, nx, nw, support, re32, im32)
+            names.append("fp32c")
+        del d3, vis3, cube32, re32, im32
         torch.cuda.empty_cache()
-
-        # chi^2 sky-model fit: 20 gradient-descent iterations through the autograd binding,
-        # each one degrid (forward) + one imaging of the weighted residuals (backward)
-        from lofar_amd.autograd import ImagingOperator, chi2_descent, lipschitz
-        op = ImagingOperator(d["uu"], d["vv"], d["ww"], None, nw, support, nx, ny,
-                             d["resolution"], d["wmin"], d["wmax"], precision="fp32c")
-        yy, xx = torch.meshgrid(torch.arange(ny, device=dev, dtype=torch.float64),
-                                torch.arange(nx, device=dev, dtype=torch.float64), indexing="ij")
-        sky = 1e-3 * torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
-        for (cy, cx, amp, sig) in ((1200.0, 900.0, 1.0, 3.0), (2900.0, 2500.0, 0.6, 5.0),
-                                   (700.0, 3300.0, 1.5, 2.0)):
-            sky += amp * torch.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sig * sig))
-        del yy, xx
-        zero = torch.zeros_like(sky)
-        step = 0.5 / lipschitz(op, d["weight"], iters=4)
-        hist = []
-        chi2_descent(op, d["vis_real"], d["vis_img"], d["weight"], sky, zero, iters=1,
-                     step=step)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        chi2_descent(op, d["vis_real"], d["vis_img"], d["weight"], sky, zero, iters=20,
-                     step=step, history=hist)
-        torch.cuda.synchronize()
-        msG = (time.perf_counter() - t0) * 1e3 / 20
-        h = [float(v) for v in hist]
-        extra["chi2_fit"] = {"workload": "20 gradient-descent iterations of the weighted chi^2 "
-                                         "of a 4096x4096 sky model against 16M samples, 32 "
-                                         "w-planes, FP32C, torch.autograd (forward degrid + "
-                                         "backward imaging of residuals)",
-                             "value": n2 / (msG * 1e-3) / 1e6, "unit": "Mvis/s per iteration",
-                             "ms_per_iteration": msG, "chi2_first": h[0], "chi2_last": h[-1],
-                             "monotone": all(b <= a * (1 + 1e-6) for a, b in zip(h, h[1:]))}
-        print("bench: chi2 fit %.3f ms/iteration, chi2 %.4e -> %.4e" % (msG, h[0], h[-1]),
-              file=sys.stderr, flush=True)
-        del op, sky, zero
+        cubeA = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
+        extra["adjoint"] = bench_adjoint(d, vis, cubeA, n2, nx, ny, nw, support, a)
+        del cubeA
+        torch.cuda.empty_cache()
+        extra["chi2_fit"] = bench_chi2(d, n2, nx, ny, nw, support)
         torch.cuda.empty_cache()
         extra["dropin"] = bench_dropin(d, n2, nx, ny, nw, support)
+        del d, vis, re64, im64
+        torch.cuda.empty_cache()
+        if not a.no_stream:
+            extra["stream"] = bench_stream(dev)
+            torch.cuda.empty_cache()
 
-    cpu = None if a.no_cpu else cpu_baseline(n2, nx, nw)
+    cpu = None
+    if names:
+        print("bench: cpu_baseline leg (reference CPU path on this host)", file=sys.stderr,
+              flush=True)
+        res = cpu_baseline(work, names)
+        wl = res.get("workloads", {})
+        f64 = wl.get("fp64", {})
+        cpu = dict(f64.get("baseline", {"error": res.get("error", "no fp64 workload")}))
+        cpu["host"] = res.get("host")
+        cpu["reference_path"] = f64.get("reference_path")
+        if "fp32c" in wl and "fp32c" in extra:
+            extra["fp32c"]["image_rel_rms_err_vs_reference"] = wl["fp32c"].get(
+                "image_rel_rms_err_vs_reference")
+            extra["fp32c"]["image_rel_max_err_vs_reference"] = wl["fp32c"].get(
+                "image_rel_max_err_vs_reference")
+    shutil.rmtree(work, ignore_errors=True)
     line = {
         "metric": METRIC, "value": value, "unit": "Mvis/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
@@ -385,6 +467,8 @@ def main():
         "config": {"workload": "FP64 end-to-end imaging (grid+FFT+phase+plane sum), 16M samples, "
                                "4096x4096 grid, 32 w-planes, support 7, inputs resident in HBM",
                    "samples": n2, "grid": [nx, ny], "w_planes": nw, "support": support},
+        "image_rel_rms_err_vs_reference": f64.get("image_rel_rms_err_vs_reference") if names else None,
+        "image_rel_max_err_vs_reference": f64.get("image_rel_max_err_vs_reference") if names else None,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -133,6 +133,36 @@ def test_fft_non_power_of_two_bruteforce():
     assert rel(got, want / (nx * ny)) < 1e-14
 
 
+@pytest.mark.parametrize("nx,ny", [(64, 32), (256, 128), (96, 60)])
+def test_reference_path_fft_matches_oracle_fft(nx, ny):
+    """oracle/refpath.py does fftw_data()'s plane transform with scipy.fft.ifft2 at the
+    headline sizes; it must agree with the oracle's own (pinned) DFT."""
+    import scipy.fft
+    from oracle import refpath
+    rng = np.random.default_rng(nx * ny)
+    cube = rng.standard_normal((2, ny, nx)) + 1j * rng.standard_normal((2, ny, nx))
+    want = O.fft_planes(cube)
+    got = cube.copy()
+    refpath.fft_reference(got, threads=2)
+    assert rel(got, want) < 1e-14
+    assert rel(scipy.fft.ifft2(cube[0]), want[0]) < 1e-14
+
+
+@needs_ref
+def test_reference_path_matches_oracle_image():
+    """The headline-size reference path (the reference's OpenMP gridder and phase loop
+    around scipy's transform) agrees with the oracle's serial pipeline."""
+    from lofar_amd.synth import make_obs
+    from oracle import refpath
+    d = make_obs(20000, 128, 4, seed=3)
+    want = O.image(4, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
+                   128, 128, d["resolution"], d["wmin"], d["wmax"])
+    got, t = refpath.reference_image(d, 4, 7, 128, 128, d["resolution"], d["wmin"], d["wmax"],
+                                     threads=2)
+    assert "reference" in t["gridder"] and "reference" in t["phase"]
+    assert rel(got, want) < 1e-12
+
+
 def test_fft_delta_closed_form():
     nx, ny = 16, 8
     cube = np.zeros((1, ny, nx), complex)
