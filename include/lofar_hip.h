@@ -30,6 +30,8 @@ extern "C" {
 #define LOFAR_EHIP (-4)    /* HIP runtime error */
 #define LOFAR_ENOMEM (-5)  /* device allocation failed */
 #define LOFAR_ENODEV (-6)  /* no usable gfx950 device */
+#define LOFAR_ESCRATCH (-7) /* the call needs more workspace than lofar_image_dev_static's
+                               static array holds */
 
 /* precision modes of the device entry points */
 #define LOFAR_FP64 0  /* double cube, double FFT, double phase and plane sum */
@@ -113,6 +115,27 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
                     double dw, int w_support, int grid_size_x, int grid_size_y, double resolution,
                     double wmin, double wmax, void* cube, double* image_real, double* image_imag,
                     int accumulate, void* stream);
+
+/* lofar_image_dev without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results
+ * as lofar_image_dev; the library's workspace (bucket tables, sorted entries,
+ * per-tile occupancy bits, twiddle tables) comes from one static device array of
+ * LOFAR_STATIC_SCRATCH_BYTES per GPU (held until the process ends) instead of
+ * hipMalloc, and twiddles are computed on the device.  A call that needs more returns
+ * LOFAR_ESCRATCH before writing the image (about 3M samples on 4096x4096 x 32 planes in
+ * FP64 mode).  The caller still provides `cube` and the image.  Every call on one GPU
+ * shares that GPU's array and leaves the last call's contents in it: calls on one device
+ * are serialised by the library (its per-device lock and stream event), and overlapping
+ * work that bypassed that ordering would race on the array. */
+#define LOFAR_STATIC_SCRATCH_BYTES (256ull << 20)
+int lofar_image_dev_static(int precision, int num_w_planes, long num_points,
+                           long freq_per_chan, long polarizations, const double* uu,
+                           const double* vv, const double* ww, const float* vis_real,
+                           const float* vis_img, const float* weight, double dx, double dw,
+                           int w_support, int grid_size_x, int grid_size_y, double resolution,
+                           double wmin, double wmax, void* cube, double* image_real,
+                           double* image_imag, int accumulate, void* stream);
 
 /* Adjoint of phase_correction(): cube[w] = image * conj(phase_w) * dwnorm*sqrt(1-l^2-m^2). */
 int lofar_phase_adjoint_dev(int precision, const double* image_real, const double* image_imag,

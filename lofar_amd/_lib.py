@@ -13,7 +13,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblofar_hip.so")
 
-OK, EARG, ECOORD, ESIZE, EHIP, ENOMEM, ENODEV = 0, -1, -2, -3, -4, -5, -6
+OK, EARG, ECOORD, ESIZE, EHIP, ENOMEM, ENODEV, ESCRATCH = 0, -1, -2, -3, -4, -5, -6, -7
+STATIC_SCRATCH_BYTES = 256 << 20  # LOFAR_STATIC_SCRATCH_BYTES
 FP64, FP32C = 0, 1
 NSTAGES = 10
 
@@ -51,6 +52,8 @@ SIGNATURES = {
     "lofar_phase_dev": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _d, _d, _d, _i, _vp]),
     "lofar_image_dev": (_i, [_i, _i, _l, _l, _l, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _i, _i,
                              _d, _d, _d, _vp, _vp, _vp, _i, _vp]),
+    "lofar_image_dev_static": (_i, [_i, _i, _l, _l, _l, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i,
+                                    _i, _i, _d, _d, _d, _vp, _vp, _vp, _i, _vp]),
     "lofar_phase_adjoint_dev": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _d, _d, _d, _vp]),
     "lofar_degrid_cube_dev": (_i, [_i, _i, _l, _l, _l, _vp, _vp, _vp, _vp, _d, _d, _i, _i, _i, _vp,
                                    _vp, _vp, _vp]),

@@ -43,6 +43,8 @@ struct DeviceCtx {
     Buffer host_io;                 // device copies of drop-in host arrays
     Buffer occ;                     // per-tile occupancy of the fused imaging path
     long long* status_host = nullptr;  // pinned: [total, err]
+    char* scratch = nullptr;        // this device's copy of `scratch` (allocation-free calls)
+    hipError_t scratch_err = hipSuccess;
     std::map<long long, void*> twiddles;
     std::vector<Prof> prof;
     std::vector<hipEvent_t> pool;
@@ -50,6 +52,14 @@ struct DeviceCtx {
 
 DeviceCtx g_ctx[64];
 std::mutex g_init_mu;
+
+// Workspace of the allocation-free entry point (lofar_image_dev_static): one static
+// device array, of which every GPU the library runs on holds its own copy until the
+// process ends.  Host code never uses the array's name as an address (on the host it is
+// a stand-in symbol); each device's copy is located once, at the context's set-up, with
+// hipGetSymbolAddress, and only that pointer reaches kernels.
+constexpr size_t kScratchBytes = LOFAR_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 bool g_profiling = false;
 
 int fail(int code)
@@ -89,6 +99,15 @@ int get_ctx(DeviceCtx** out)
         if (hipHostMalloc((void**)&c.status_host, 2 * sizeof(long long), hipHostMallocDefault) !=
             hipSuccess)
             return LOFAR_EHIP;
+        // this device's copy of the allocation-free workspace (device `dev` is current)
+        void* sp = nullptr;
+        c.scratch_err = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (c.scratch_err == hipSuccess) {
+            c.scratch = (char*)sp;
+        } else {
+            (void)hipGetLastError();
+            c.scratch = nullptr;
+        }
         c.device = dev;
         c.ready = true;
     }
@@ -111,6 +130,37 @@ int ensure(DeviceCtx& c, Buffer& b, size_t bytes)
         return fail(LOFAR_ENOMEM);
     }
     b.size = want;
+    return LOFAR_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Where a call's workspace comes from: the context's grow-only buffers (base ==
+// nullptr), or consecutive 256-byte-aligned slices of the static `scratch` array
+// (the allocation-free entry point), which never grows.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+int take(DeviceCtx& c, Arena* ar, Buffer& b, size_t bytes, char** out)
+{
+    if (!ar || !ar->base) {
+        int r = ensure(c, b, bytes);
+        if (r) return r;
+        *out = (char*)b.ptr;
+        return LOFAR_OK;
+    }
+    const size_t at = align256(ar->used);
+    if (at + bytes > ar->cap) {
+        fprintf(stderr,
+                "liblofar_hip: lofar_image_dev_static needs more than its %zu-byte static "
+                "workspace (%zu bytes so far); use lofar_image_dev for this size\n",
+                ar->cap, at + bytes);
+        return fail(LOFAR_ESCRATCH);
+    }
+    *out = ar->base + at;
+    ar->used = at + bytes;
     return LOFAR_OK;
 }
 
@@ -142,6 +192,20 @@ int twiddles(DeviceCtx& c, int n, int precision, void** out)
     }
     c.twiddles[key] = d;
     *out = d;
+    return LOFAR_OK;
+}
+
+// Twiddles of the allocation-free entry point: a slice of the static workspace
+// filled on the device (no allocation and no host copy).
+int twiddles_for(DeviceCtx& c, Arena* ar, int n, int precision, hipStream_t st, void** out)
+{
+    if (!ar || !ar->base) return twiddles(c, n, precision, out);
+    Buffer unused;
+    char* p = nullptr;
+    int r = take(c, ar, unused, (size_t)n * (precision == LOFAR_FP64 ? 16 : 8), &p);
+    if (r) return r;
+    HIPCHK(fill_twiddles(precision, n, p, st));
+    *out = p;
     return LOFAR_OK;
 }
 
@@ -204,8 +268,6 @@ Geometry make_geometry(int nw, int support, int nx, int ny, double dx, double dw
     return g;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 // Bucket-sort tuning: super-tile side (tiles), samples per level-1 workgroup
 // and records per level-2 block.  LOFAR_BUCKET_CFG="sst,chunk,block" overrides,
 // for measurement.
@@ -255,7 +317,7 @@ constexpr long long kMaxSuperTiles = 8192;
 
 int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double* vv,
                   const double* ww, const Geometry& g, hipStream_t st, Buckets& b,
-                  long long& total)
+                  long long& total, Arena* ar = nullptr)
 {
     const BucketCfg cfg = bucket_cfg();
     int sst = cfg.sst;
@@ -299,9 +361,9 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     const size_t off_l2nblk = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
     const size_t off_l2boff = align256(off_l2nblk + (size_t)m.nk1 * sizeof(unsigned));
     const size_t bytes = align256(off_l2boff + (size_t)(m.nk1 + 1) * sizeof(long long));
-    int r = ensure(c, c.meta, bytes);
+    char* base = nullptr;
+    int r = take(c, ar, c.meta, bytes, &base);
     if (r) return r;
-    char* base = (char*)c.meta.ptr;
     b.total = (long long*)(base + off_status);
     b.err = (int*)(base + off_status + sizeof(long long));
     b.total_rec = (long long*)(base + off_totrec);
@@ -327,7 +389,8 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
 int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long F, long P,
               const double* uu, const double* vv, const double* ww, const float* vr,
               const float* vi, const float* wt, double dx, double dw, int support, int nx, int ny,
-              void* cube, hipStream_t st, unsigned long long* occ = nullptr, int sm_log = -1)
+              void* cube, hipStream_t st, unsigned long long* occ = nullptr, int sm_log = -1,
+              Arena* ar = nullptr)
 {
     int r = check_grid_args(precision, nw, n, F, P, dx, dw, support, nx, ny);
     if (r) return fail(r);
@@ -335,7 +398,7 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     const Geometry g = make_geometry(nw, support, nx, ny, dx, dw);
     Buckets b;
     long long total = 0;
-    r = build_buckets(c, kBucketFootprint, n, uu, vv, ww, g, st, b, total);
+    r = build_buckets(c, kBucketFootprint, n, uu, vv, ww, g, st, b, total, ar);
     if (r) return r;
     // level-1 records (at most one per entry), the sorted entries, then the
     // level-2 block table
@@ -343,9 +406,9 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     const size_t rbytes = align256((size_t)total * sizeof(Entry64));
     const size_t ebytes = align256((size_t)total * esz);
     const size_t l2 = level2_layout(b, total, nullptr);
-    r = ensure(c, c.ents, rbytes + ebytes + l2 + 256);
+    char* eb = nullptr;
+    r = take(c, ar, c.ents, rbytes + ebytes + l2 + 256, &eb);
     if (r) return r;
-    char* eb = (char*)c.ents.ptr;
     Entry64* rec = (Entry64*)eb;
     void* ents = eb + rbytes;
     level2_layout(b, total, eb + rbytes + ebytes);
@@ -473,6 +536,7 @@ const char* lofar_error_string(int code)
         case LOFAR_EHIP: return "HIP runtime error";
         case LOFAR_ENOMEM: return "device allocation failed";
         case LOFAR_ENODEV: return "no usable gfx950 device";
+        case LOFAR_ESCRATCH: return "workspace exceeds the allocation-free entry point's static array";
         default: return "unknown error";
     }
 }
@@ -570,18 +634,16 @@ int lofar_phase_adjoint_dev(int precision, const double* image_real, const doubl
     return LOFAR_OK;
 }
 
-int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,
-                    long polarizations, const double* uu, const double* vv, const double* ww,
-                    const float* vis_real, const float* vis_img, const float* weight, double dx,
-                    double dw, int w_support, int grid_size_x, int grid_size_y, double resolution,
-                    double wmin, double wmax, void* cube, double* image_real, double* image_imag,
-                    int accumulate, void* stream)
+// The fused imaging pipeline of lofar_image_dev and lofar_image_dev_static; `ar`
+// selects where the workspace comes from (nullptr: the context's buffers).
+static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, int num_w_planes,
+                      long num_points, long freq_per_chan, long polarizations, const double* uu,
+                      const double* vv, const double* ww, const float* vis_real,
+                      const float* vis_img, const float* weight, double dx, double dw,
+                      int w_support, int grid_size_x, int grid_size_y, double resolution,
+                      double wmin, double wmax, void* cube, double* image_real,
+                      double* image_imag, int accumulate)
 {
-    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
-    if (!image_real || !image_imag) return fail(LOFAR_EARG);
-    CallGuard cg(stream);
-    if (cg.status) return cg.status;
-    DeviceCtx& c = *cg.c;
     // per-tile occupancy bits: the gridder skips empty tiles and the column pass
     // reads them as zeros (through 32-bit plane offsets, so for planes below 2 GiB;
     // larger planes are written whole)
@@ -593,10 +655,11 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
         const long long ntv = (grid_size_y + kTile - 1) / kTile;
         const long long ntu = (grid_size_x + kTile - 1) / kTile;
         const long long nocc = (long long)num_w_planes * ntu * ((ntv + 63) / 64);
-        r = ensure(c, c.occ, (size_t)nocc * 8 + 256);
+        char* op = nullptr;
+        r = take(c, ar, c.occ, (size_t)nocc * 8 + 256, &op);
         if (r) return r;
-        occ = (unsigned long long*)c.occ.ptr;
-        HIPCHK(fill_u32((unsigned*)occ, 2 * nocc, 0u, cg.st));
+        occ = (unsigned long long*)op;
+        HIPCHK(fill_u32((unsigned*)occ, 2 * nocc, 0u, st));
     }
     // FP64: the scratch cube is laid out strip-major, in strips of 2^sm_log whole
     // columns (the 64-column tile width by default), so every tile flush is one
@@ -611,51 +674,94 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
                            : -1;
     r = grid_impl(c, precision, 0, num_w_planes, num_points, freq_per_chan, polarizations, uu, vv,
                   ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x, grid_size_y, cube,
-                  cg.st, occ, sm_log);
+                  st, occ, sm_log, ar);
     if (r) return r;
     void *twx, *twy;
-    r = twiddles(c, grid_size_x, precision, &twx);
+    r = twiddles_for(c, ar, grid_size_x, precision, st, &twx);
     if (r) return r;
-    r = twiddles(c, grid_size_y, precision, &twy);
+    r = twiddles_for(c, ar, grid_size_y, precision, st, &twy);
     if (r) return r;
     const PhaseParams pp =
         phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
     const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
     if (precision == LOFAR_FP64) {
-        HIPCHK(stage(c, kStCols, cg.st, [&] {
+        HIPCHK(stage(c, kStCols, st, [&] {
             return fft_cols<double>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                    1.0, cg.st, occ, sm_log);
+                                    1.0, st, occ, sm_log);
         }));
-        HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
+        HIPCHK(stage(c, kStRowsPhase, st, [&] {
             return rows_phase<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
-                                      scale, image_real, image_imag, accumulate, cg.st, sm_log);
+                                      scale, image_real, image_imag, accumulate, st, sm_log);
         }));
     } else {
-        HIPCHK(stage(c, kStCols, cg.st, [&] {
+        HIPCHK(stage(c, kStCols, st, [&] {
             return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                   1.0, cg.st, occ, sm_log);
+                                   1.0, st, occ, sm_log);
         }));
         if (grid_size_x >= 8192 && sm_log < 0 && rows_split()) {
             // rows of 8192 run the fused pass at 1024 threads a workgroup, where its
             // per-pixel state does not fit the registers: the row transform (in
             // place) and the phase/plane sum run as two passes instead
-            HIPCHK(stage(c, kStRows, cg.st, [&] {
+            HIPCHK(stage(c, kStRows, st, [&] {
                 return fft_rows<float>(1, cube, cube, twx, grid_size_x,
-                                       (long long)num_w_planes * grid_size_y, scale, cg.st);
+                                       (long long)num_w_planes * grid_size_y, scale, st);
             }));
-            HIPCHK(stage(c, kStPhase, cg.st, [&] {
+            HIPCHK(stage(c, kStPhase, st, [&] {
                 return phase_sum(LOFAR_FP32C, cube, image_real, image_imag, grid_size_x,
                                  grid_size_y, num_w_planes, grid_size_x, grid_size_y, resolution,
-                                 wmin, wmax, accumulate, cg.st);
+                                 wmin, wmax, accumulate, st);
             }));
         } else {
-            HIPCHK(stage(c, kStRowsPhase, cg.st, [&] {
+            HIPCHK(stage(c, kStRowsPhase, st, [&] {
                 return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
-                                         scale, image_real, image_imag, accumulate, cg.st, sm_log);
+                                         scale, image_real, image_imag, accumulate, st, sm_log);
             }));
         }
     }
     return LOFAR_OK;
+}
+
+int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,
+                    long polarizations, const double* uu, const double* vv, const double* ww,
+                    const float* vis_real, const float* vis_img, const float* weight, double dx,
+                    double dw, int w_support, int grid_size_x, int grid_size_y, double resolution,
+                    double wmin, double wmax, void* cube, double* image_real, double* image_imag,
+                    int accumulate, void* stream)
+{
+    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    if (!image_real || !image_imag) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    return image_impl(*cg.c, cg.st, nullptr, precision, num_w_planes, num_points, freq_per_chan,
+                      polarizations, uu, vv, ww, vis_real, vis_img, weight, dx, dw, w_support,
+                      grid_size_x, grid_size_y, resolution, wmin, wmax, cube, image_real,
+                      image_imag, accumulate);
+}
+
+int lofar_image_dev_static(int precision, int num_w_planes, long num_points,
+                           long freq_per_chan, long polarizations, const double* uu,
+                           const double* vv, const double* ww, const float* vis_real,
+                           const float* vis_img, const float* weight, double dx, double dw,
+                           int w_support, int grid_size_x, int grid_size_y, double resolution,
+                           double wmin, double wmax, void* cube, double* image_real,
+                           double* image_imag, int accumulate, void* stream)
+{
+    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    if (!image_real || !image_imag) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    if (!cg.c->scratch) {
+        fprintf(stderr, "liblofar_hip: static workspace unavailable on device %d: %s\n",
+                cg.c->device, hipGetErrorString(cg.c->scratch_err));
+        return fail(LOFAR_EHIP);
+    }
+    Arena ar;
+    ar.base = cg.c->scratch;
+    ar.cap = kScratchBytes;
+    return image_impl(*cg.c, cg.st, &ar, precision, num_w_planes, num_points, freq_per_chan,
+                      polarizations, uu, vv, ww, vis_real, vis_img, weight, dx, dw, w_support,
+                      grid_size_x, grid_size_y, resolution, wmin, wmax, cube, image_real,
+                      image_imag, accumulate);
 }
 
 int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,

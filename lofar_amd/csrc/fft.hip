@@ -650,6 +650,26 @@ hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
 #undef CALL_
 }
 
+template <typename Real>
+__global__ __launch_bounds__(256) void k_twiddles(C2<Real>* tw, int n)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)n, &sn, &cs);  // m/n exact: n is a power of two
+    tw[m] = C2<Real>{(Real)cs, (Real)sn};
+}
+
+hipError_t fill_twiddles(int precision, int n, void* tw, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (precision == LOFAR_FP64)
+        hipLaunchKernelGGL((k_twiddles<double>), dim3(blocks), dim3(256), 0, st, (double2*)tw, n);
+    else
+        hipLaunchKernelGGL((k_twiddles<float>), dim3(blocks), dim3(256), 0, st, (float2*)tw, n);
+    return hipGetLastError();
+}
+
 int device_cus()
 {
     static int cus = 0;

@@ -451,6 +451,10 @@ __device__ __forceinline__ void sincos_w(double x, double* sn, double* cs)
 // phase constants of phase_correction.cu:92-94 (phase.hip)
 PhaseParams phase_params(int nw, int xtot, int ytot, double resolution, double wmin, double wmax);
 
+// tw[m] = exp(+2*pi*i*m/n), m < n, computed on the device (sincospi in double, then
+// rounded to float for LOFAR_FP32C): the allocation-free entry point's twiddle tables
+hipError_t fill_twiddles(int precision, int n, void* tw, hipStream_t st);
+
 // host dispatch (fft.hip); all return hipErrorInvalidValue for unsupported sizes
 template <typename Real>
 hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx, long long lines,

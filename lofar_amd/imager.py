@@ -198,8 +198,11 @@ def phase(cube, resolution, wmin, wmax, image_real=None, image_imag=None, accumu
 
 def image(uu, vv, ww, vis_real, vis_img, weight, num_w_planes, w_support, grid_size_x,
           grid_size_y, resolution, wmin, wmax, precision="fp64", cube=None, image_real=None,
-          image_imag=None, accumulate=False, freq_per_chan=1, polarizations=1):
-    """Whole hot path on the device: grid -> per-plane FFT -> phase correction + plane sum."""
+          image_imag=None, accumulate=False, freq_per_chan=1, polarizations=1,
+          static_workspace=False):
+    """Whole hot path on the device: grid -> per-plane FFT -> phase correction + plane sum.
+    ``static_workspace=True`` calls lofar_image_dev_static: the same pipeline with the
+    library's workspace in its static per-device array instead of hipMalloc."""
     torch = _torch()
     p = _prec(precision)
     dev = uu.device
@@ -210,7 +213,8 @@ def image(uu, vv, ww, vis_real, vis_img, weight, num_w_planes, w_support, grid_s
         image_real = torch.empty((grid_size_y, grid_size_x), dtype=torch.float64, device=dev)
         image_imag = torch.empty((grid_size_y, grid_size_x), dtype=torch.float64, device=dev)
         accumulate = False
-    check("lofar_image_dev", lib().lofar_image_dev(
+    name = "lofar_image_dev_static" if static_workspace else "lofar_image_dev"
+    check(name, getattr(lib(), name)(
         p, num_w_planes, uu.numel(), freq_per_chan, polarizations,
         *_vis_args(uu, vv, ww, vis_real, vis_img, weight),
         1.0 / grid_size_x, 1.0 / num_w_planes, w_support, grid_size_x, grid_size_y,
