@@ -121,6 +121,44 @@ def test_grid_is_bitwise_order_independent(cuda, precision, tol):
     assert st == O.OK and rel(a, want) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_grid_infinite_visibilities(cuda, precision, tol):
+    """Infinite visibilities (real parts finite, so not skipped as NaN) propagate into their
+    windows as in the reference: +inf, -inf, or NaN where both meet; the tiles holding them
+    take the floating-point (ds_add_f64) path, the others the fixed-point one."""
+    from lofar_amd import imager
+    d = random_case(31, 20000, 256, 256, 3, edge=False, nan=0)
+    vi = d["vis_img"].copy()
+    vi[[17, 4000, 4001, 12345]] = [np.inf, -np.inf, np.inf, -np.inf]
+    d["vis_img"] = vi
+    got = H(imager.grid(*dev_inputs(d, cuda), 3, 7, 256, 256, precision=precision))
+    st, want = O.wstack(3, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / 256, 1 / 3, 7, 256, 256)
+    assert st == O.OK
+    for part in (np.real, np.imag):
+        a, b = part(got), part(want)
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(np.isposinf(a), np.isposinf(b))
+        assert np.array_equal(np.isneginf(a), np.isneginf(b))
+    fin = np.isfinite(want.real) & np.isfinite(want.imag)
+    assert fin.sum() > 0.9 * want.size
+    assert rel(got[fin], want[fin]) < tol
+
+
+@pytest.mark.parametrize("grid_cfg", ["1"])
+def test_grid_accumulation_variants(cuda, grid_cfg):
+    """The floating-point accumulation path (LOFAR_GRID_CFG=1, read once per process: LDS
+    ds_add_f64 in both modes instead of exact fixed point) gives the oracle's cube, in a
+    child process with the setting applied."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, LOFAR_GRID_CFG=grid_cfg)
+    r = subprocess.run([sys.executable, "-c", _BUCKET_SCRIPT, root], env=env, cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "BUCKETS_OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
 _BUCKET_SCRIPT = r"""
 import sys, numpy as np, torch
 sys.path.insert(0, sys.argv[1])
