@@ -45,16 +45,22 @@ __device__ __forceinline__ TileSpan tiles_of(const Window& s)
 // ---------------------------------------------------------------------------
 // level 1: count
 
+// GLOBAL: more super-tiles than the LDS histograms hold; the counts go straight to
+// count1/count1e with global atomics (no per-workgroup bases).
+template <bool GLOBAL>
 __global__ __launch_bounds__(256) void k_l1_count(int mode, long long n, const double* uu,
                                                   const double* vv, const double* ww, Geometry g,
                                                   TileMap tm, long long chunk, unsigned* count1,
                                                   unsigned* count1e, unsigned* base1, int* err)
 {
-    extern __shared__ unsigned hist[];  // [nk1] records, then [nk1] entries
-    unsigned* hent = hist + tm.nk1;
+    extern __shared__ unsigned hist_lds[];  // [nk1] records, then [nk1] entries
+    unsigned* hist = GLOBAL ? count1 : hist_lds;
+    unsigned* hent = GLOBAL ? count1e : hist_lds + tm.nk1;
     const int tid = threadIdx.x;
-    for (long long k = tid; k < 2 * tm.nk1; k += 256) hist[k] = 0u;
-    __syncthreads();
+    if constexpr (!GLOBAL) {
+        for (long long k = tid; k < 2 * tm.nk1; k += 256) hist[k] = 0u;
+        __syncthreads();
+    }
     const long long beg = (long long)blockIdx.x * chunk;
     const long long end = min(n, beg + chunk);
     bool bad = false;
@@ -91,6 +97,7 @@ __global__ __launch_bounds__(256) void k_l1_count(int mode, long long n, const d
       }
     }
     if (bad) atomicOr(err, 1);
+    if constexpr (GLOBAL) return;
     __syncthreads();
     unsigned* my_base = base1 + (long long)blockIdx.x * tm.nk1;
     for (long long k = tid; k < tm.nk1; k += 256) {
@@ -172,47 +179,26 @@ __global__ __launch_bounds__(256) void k_scan_add(long long* out, const long lon
 // ---------------------------------------------------------------------------
 // level 1: scatter records into super-tile order
 
-// Channel/polarisation sum of one sample with the reference's weighting:
-// float product weight*vis, promoted to double, NaN real parts skipped
-// (w-stacking_omp.c:116-133).
-__device__ __forceinline__ void sample_vis(long long i, long long F, long long P, const float* vr,
-                                           const float* vi, const float* wt, double& sr,
-                                           double& si)
-{
-    sr = 0.0;
-    si = 0.0;
-    const long long vbase = i * F * P;
-    for (long long f = 0; f < F; f++) {
-        for (long long p = 0; p < P; p++) {
-            const long long ifine = vbase + f * P + p;
-            const float x = vr[ifine];
-            if (!isnan(x)) {
-                const float w = wt[i * P + p];
-                const float a = w * x;
-                const float b = w * vi[ifine];
-                sr += (double)a;
-                si += (double)b;
-            }
-        }
-    }
-}
-
 // Rec: Entry64 (FP64 mode) or Rec32 (FP32 mode, position relative to the
 // super-tile, rec_axis)
-template <typename Rec>
+// GLOBAL: base1 is one global cursor per super-tile (zeroed), taken with global atomics.
+template <typename Rec, bool GLOBAL>
 __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long F, long long P,
                                                          const double* uu, const double* vv,
                                                          const double* ww, const float* vr,
                                                          const float* vi, const float* wt,
                                                          Geometry g, TileMap tm, long long chunk,
                                                          const long long* off1,
-                                                         const unsigned* base1, Rec* rec)
+                                                         unsigned* base1, Rec* rec)
 {
-    extern __shared__ unsigned cur[];
+    extern __shared__ unsigned cur_lds[];
+    unsigned* cur = GLOBAL ? base1 : cur_lds;
     const int tid = threadIdx.x;
-    const unsigned* my_base = base1 + (long long)blockIdx.x * tm.nk1;
-    for (long long k = tid; k < tm.nk1; k += 256) cur[k] = my_base[k];
-    __syncthreads();
+    if constexpr (!GLOBAL) {
+        const unsigned* my_base = base1 + (long long)blockIdx.x * tm.nk1;
+        for (long long k = tid; k < tm.nk1; k += 256) cur[k] = my_base[k];
+        __syncthreads();
+    }
     const long long beg = (long long)blockIdx.x * chunk;
     const long long end = min(n, beg + chunk);
     // kUnroll samples per thread per round, their loads issued together
@@ -252,17 +238,21 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
     }
 }
 
+template <bool GLOBAL>
 __global__ __launch_bounds__(256) void k_l1_scatter_home(long long n, const double* uu,
                                                          const double* vv, const double* ww,
                                                          Geometry g, TileMap tm, long long chunk,
                                                          const long long* off1,
-                                                         const unsigned* base1, DEntry* rec)
+                                                         unsigned* base1, DEntry* rec)
 {
-    extern __shared__ unsigned cur[];
+    extern __shared__ unsigned cur_lds[];
+    unsigned* cur = GLOBAL ? base1 : cur_lds;
     const int tid = threadIdx.x;
-    const unsigned* my_base = base1 + (long long)blockIdx.x * tm.nk1;
-    for (long long k = tid; k < tm.nk1; k += 256) cur[k] = my_base[k];
-    __syncthreads();
+    if constexpr (!GLOBAL) {
+        const unsigned* my_base = base1 + (long long)blockIdx.x * tm.nk1;
+        for (long long k = tid; k < tm.nk1; k += 256) cur[k] = my_base[k];
+        __syncthreads();
+    }
     const long long beg = (long long)blockIdx.x * chunk;
     const long long end = min(n, beg + chunk);
     for (long long i = beg + tid; i < end; i += 256) {
@@ -504,8 +494,11 @@ hipError_t bucket_count(int mode, long long n, const double* uu, const double* v
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(b.map.nk1, 256)), dim3(256), 0, st, b.count1e,
                        b.map.nk1, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (unsigned*)b.err, 1LL, 0u);
-    if (n > 0)
-        hipLaunchKernelGGL(k_l1_count, dim3((unsigned)b.nwg), dim3(256),
+    if (n > 0 && b.global_l1)
+        hipLaunchKernelGGL(k_l1_count<true>, dim3((unsigned)b.nwg), dim3(256), 0, st, mode, n, uu,
+                           vv, ww, g, b.map, b.chunk, b.count1, b.count1e, b.base1, b.err);
+    else if (n > 0)
+        hipLaunchKernelGGL(k_l1_count<false>, dim3((unsigned)b.nwg), dim3(256),
                            2 * (size_t)b.map.nk1 * sizeof(unsigned), st, mode, n, uu, vv, ww, g,
                            b.map, b.chunk, b.count1, b.count1e, b.base1, b.err);
     return hipGetLastError();
@@ -544,18 +537,25 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
                             Entry64* rec, void* entries, hipStream_t st)
 {
+    const size_t lds = b.global_l1 ? 0 : (size_t)b.map.nk1 * sizeof(unsigned);
+    if (n > 0 && b.global_l1) {
+        hipError_t e = fill_u32(b.base1, b.map.nk1, 0u, st);
+        if (e != hipSuccess) return e;
+    }
     if (precision == LOFAR_FP64) {
-        if (n > 0)
-            hipLaunchKernelGGL((k_l1_scatter_grid<Entry64>), dim3((unsigned)b.nwg), dim3(256),
-                               (size_t)b.map.nk1 * sizeof(unsigned), st, n, F, P, uu, vv, ww, vr,
-                               vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec);
+        if (n > 0) {
+            auto k = b.global_l1 ? k_l1_scatter_grid<Entry64, true> : k_l1_scatter_grid<Entry64, false>;
+            hipLaunchKernelGGL(k, dim3((unsigned)b.nwg), dim3(256), lds, st, n, F, P, uu, vv, ww,
+                               vr, vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec);
+        }
         return level2<kOutFp64>(g, b, (const Entry64*)rec, (Entry64*)entries, st);
     }
     Rec32* rec32 = (Rec32*)rec;  // the record buffer holds total_rec 32-byte records
-    if (n > 0)
-        hipLaunchKernelGGL((k_l1_scatter_grid<Rec32>), dim3((unsigned)b.nwg), dim3(256),
-                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, F, P, uu, vv, ww, vr, vi,
-                           wt, g, b.map, b.chunk, b.off1, b.base1, rec32);
+    if (n > 0) {
+        auto k = b.global_l1 ? k_l1_scatter_grid<Rec32, true> : k_l1_scatter_grid<Rec32, false>;
+        hipLaunchKernelGGL(k, dim3((unsigned)b.nwg), dim3(256), lds, st, n, F, P, uu, vv, ww, vr,
+                           vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec32);
+    }
     return level2<kOutFp32>(g, b, (const Rec32*)rec32, (Entry32*)entries, st);
 }
 
@@ -563,10 +563,16 @@ hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, con
                             const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
                             hipStream_t st)
 {
-    if (n > 0)
-        hipLaunchKernelGGL(k_l1_scatter_home, dim3((unsigned)b.nwg), dim3(256),
+    if (n > 0 && b.global_l1) {
+        hipError_t e = fill_u32(b.base1, b.map.nk1, 0u, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_l1_scatter_home<true>, dim3((unsigned)b.nwg), dim3(256), 0, st, n,
+                           uu, vv, ww, g, b.map, b.chunk, b.off1, b.base1, rec);
+    } else if (n > 0) {
+        hipLaunchKernelGGL(k_l1_scatter_home<false>, dim3((unsigned)b.nwg), dim3(256),
                            (size_t)b.map.nk1 * sizeof(unsigned), st, n, uu, vv, ww, g, b.map,
                            b.chunk, b.off1, b.base1, rec);
+    }
     return level2<kOutHome>(g, b, (const DEntry*)rec, entries, st);
 }
 

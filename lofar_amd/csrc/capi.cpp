@@ -248,7 +248,7 @@ int check_grid_args(int precision, int nw, long n, long F, long P, double dx, do
                     int support, int nx, int ny)
 {
     if (precision != LOFAR_FP64 && precision != LOFAR_FP32C) return LOFAR_EARG;
-    if (nw < 1 || n < 0 || F < 1 || P < 1 || support < 1 || support > 17) return LOFAR_EARG;
+    if (nw < 1 || n < 0 || F < 1 || P < 1 || support < 1 || support > 65535) return LOFAR_EARG;
     if (nx < 1 || ny < 1 || nx > 65536 || ny > 65536) return LOFAR_EARG;
     if (!(dx > 0.0) || !(dw > 0.0) || !std::isfinite(dx) || !std::isfinite(dw)) return LOFAR_EARG;
     return LOFAR_OK;
@@ -312,8 +312,21 @@ size_t level2_layout(Buckets& b, long long total, char* base)
 // workspace, counts per super-tile, scans, and reads back the entry total and
 // the validation flag (the only host synchronisation of a call).  The level-1
 // histograms of nw*ceil(ntv/sst)*ceil(ntu/sst) super-tiles (records and
-// entries) live in LDS and are limited to kMaxSuperTiles (64 KiB).
+// entries) live in LDS up to kMaxSuperTiles (64 KiB), in global memory beyond.
 constexpr long long kMaxSuperTiles = 8192;
+
+// LDS histogram capacity in super-tiles; LOFAR_L1_LDS_MAX lowers it (tests of the
+// global-histogram path on small grids)
+long long l1_lds_max()
+{
+    static long long v = -1;
+    if (v < 0) {
+        const char* e = getenv("LOFAR_L1_LDS_MAX");
+        v = e ? atoll(e) : kMaxSuperTiles;
+        if (v < 1 || v > kMaxSuperTiles) v = kMaxSuperTiles;
+    }
+    return v;
+}
 
 int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double* vv,
                   const double* ww, const Geometry& g, hipStream_t st, Buckets& b,
@@ -337,13 +350,9 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     m.nstv = (g.ntv + sst - 1) / sst;
     m.rfb = rec_fraction_bits(sst);
     m.nk1 = super_tiles(sst);
-    if (m.nk1 > kMaxSuperTiles) {
-        fprintf(stderr,
-                "liblofar_hip: %d w-planes of %dx%d cells need %lld super-tiles of %dx%d uv "
-                "tiles; the bucket sort handles at most %lld\n",
-                g.nw, g.nx, g.ny, m.nk1, sst, sst, kMaxSuperTiles);
-        return fail(LOFAR_ESIZE);
-    }
+    // beyond the LDS histograms' capacity, level 1 counts and places records with global
+    // atomics on per-super-tile counters (slower, no size limit)
+    b.global_l1 = m.nk1 > l1_lds_max();
     b.nwg = n > 0 ? (n + chunk - 1) / chunk : 1;
     if (b.nwg > 4096) b.nwg = 4096;
     b.chunk = n > 0 ? (n + b.nwg - 1) / b.nwg : 1;
@@ -355,7 +364,8 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     const size_t off_count1e = align256(off_off1 + (size_t)(m.nk1 + 1) * sizeof(long long));
     const size_t off_eoff1 = align256(off_count1e + (size_t)m.nk1 * sizeof(unsigned));
     const size_t off_base1 = align256(off_eoff1 + (size_t)(m.nk1 + 1) * sizeof(long long));
-    const size_t off_offsets = align256(off_base1 + (size_t)(b.nwg * m.nk1) * sizeof(unsigned));
+    const size_t nbase1 = b.global_l1 ? (size_t)m.nk1 : (size_t)(b.nwg * m.nk1);
+    const size_t off_offsets = align256(off_base1 + nbase1 * sizeof(unsigned));
     const size_t off_partials =
         align256(off_offsets + (size_t)(m.nbuckets() + 1) * sizeof(long long));
     const size_t off_l2nblk = align256(off_partials + (size_t)(nblk + 1) * sizeof(long long));
@@ -386,6 +396,21 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     return LOFAR_OK;
 }
 
+// Sample validation of the bucket-free path (the checks of the bucket count): read back
+// before anything is written, so that a bad sample leaves the outputs untouched.
+int validate_direct(DeviceCtx& c, Arena* ar, long n, const double* uu, const double* vv,
+                    const double* ww, const Geometry& g, hipStream_t st)
+{
+    char* p = nullptr;
+    int r = take(c, ar, c.meta, 256, &p);
+    if (r) return r;
+    HIPCHK(validate_samples(n, uu, vv, ww, g, (int*)p, st));
+    HIPCHK(hipMemcpyAsync(&c.status_host[1], p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if ((int)(c.status_host[1] & 0xffffffff)) return fail(LOFAR_ECOORD);
+    return LOFAR_OK;
+}
+
 int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long F, long P,
               const double* uu, const double* vv, const double* ww, const float* vr,
               const float* vi, const float* wt, double dx, double dw, int support, int nx, int ny,
@@ -396,6 +421,18 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     if (r) return fail(r);
     if (!cube || (n > 0 && (!uu || !vv || !ww || !vr || !vi || !wt))) return fail(LOFAR_EARG);
     const Geometry g = make_geometry(nw, support, nx, ny, dx, dw);
+    if (2 * g.K + 1 > kMaxTileWin) {
+        // windows wider than the tile kernels take: validated, then gridded with
+        // global atomics, no buckets; the cube is written whole (no occupancy bits)
+        r = validate_direct(c, ar, n, uu, vv, ww, g, st);
+        if (r) return r;
+        if (occ || sm_log >= 0) return fail(LOFAR_EARG);
+        HIPCHK(stage(c, kStGrid, st, [&] {
+            return grid_direct(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, cube, accumulate,
+                               st);
+        }));
+        return LOFAR_OK;
+    }
     Buckets b;
     long long total = 0;
     r = build_buckets(c, kBucketFootprint, n, uu, vv, ww, g, st, b, total, ar);
@@ -651,7 +688,10 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
     unsigned long long* occ = nullptr;
     const long long plane_bytes = (long long)grid_size_x * grid_size_y *
                                   (precision == LOFAR_FP64 ? 16 : 8);
-    if (plane_bytes < (1LL << 31)) {
+    // windows wider than the tile kernels take are gridded without buckets into a whole
+    // row-major cube
+    const bool direct = w_support >= 1 && 2 * ((w_support - 1) / 2) + 1 > kMaxTileWin;
+    if (plane_bytes < (1LL << 31) && !direct) {
         const long long ntv = (grid_size_y + kTile - 1) / kTile;
         const long long ntu = (grid_size_x + kTile - 1) / kTile;
         const long long nocc = (long long)num_w_planes * ntu * ((ntv + 63) / 64);
@@ -666,7 +706,7 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
     // contiguous block and the column transforms read 64-byte runs 1 KiB apart instead
     // of a plane row apart; FP32C keeps row-major planes (no gain measured there).
     // LOFAR_SM_LOG overrides, for measurement (-1: row-major; at most the tile width).
-    const int sm_pref = sm_log_cfg(precision);
+    const int sm_pref = direct ? -1 : sm_log_cfg(precision);
     const int sm_log = (sm_pref >= 0 && (1 << sm_pref) <= kTile &&
                         grid_size_x % (1 << sm_pref) == 0 &&
                         (1 << sm_pref) % (precision == LOFAR_FP64 ? 4 : 8) == 0)
@@ -780,6 +820,15 @@ int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long
     DeviceCtx& c = *cg.c;
     const Geometry g =
         make_geometry(num_w_planes, w_support, grid_size_x, grid_size_y, dx, dw);
+    if (2 * g.K + 1 > kMaxTileWin) {
+        r = validate_direct(c, nullptr, num_points, uu, vv, ww, g, cg.st);
+        if (r) return r;
+        HIPCHK(stage(c, kStDegrid, cg.st, [&] {
+            return degrid_direct(precision, num_points, freq_per_chan, polarizations, uu, vv, ww,
+                                 weight, g, cube, model_real, model_imag, cg.st);
+        }));
+        return LOFAR_OK;
+    }
     Buckets b;
     long long total = 0;
     r = build_buckets(c, kBucketHome, num_points, uu, vv, ww, g, cg.st, b, total);

@@ -13,8 +13,9 @@ namespace lofar {
 
 // Side of the square uv tile one workgroup owns while gridding.
 constexpr int kTile = 64;
-// Largest Gaussian window (2K+1) handled by the unrolled gridding kernels.
-constexpr int kMaxWin = 9;
+// Largest Gaussian window (2K+1) the tile kernels take; wider windows use the
+// bucket-free direct kernels (grid_direct, degrid_direct).
+constexpr int kMaxTileWin = 17;
 
 template <typename T> struct cplx;
 template <> struct cplx<double> { using type = double2; };

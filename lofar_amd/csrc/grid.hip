@@ -21,6 +21,11 @@ namespace lofar {
 // ---------------------------------------------------------------------------
 // gridding
 
+__global__ __launch_bounds__(64) void k_fill_u32_1(unsigned* p)
+{
+    if (threadIdx.x == 0) *p = 0u;
+}
+
 // Window of an entry at truncated cell (gu, gv), clipped to the grid as
 // w-stacking_omp.c:93-96 does and then to the tile at (u0, v0): first cell
 // (a0, b0) and inclusive extents (na, nb), tile-local.
@@ -751,6 +756,84 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
 }
 
 // ---------------------------------------------------------------------------
+// bucket-free kernels for windows wider than the tile kernels take
+
+__global__ __launch_bounds__(256) void k_validate(long long n, const double* uu, const double* vv,
+                                                  const double* ww, Geometry g, int* err)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SampleWin s;
+    if (!sample_window(uu[i], vv[i], ww[i], g, s)) atomicOr(err, 1);
+}
+
+// One thread per sample adds weight*vis*norm*exp(-(du^2+dv^2)/2) to every cell of its
+// clipped window with global atomic adds, as the reference's OpenMP loop does
+// (w-stacking.cu:222-307, `omp atomic` on each cell); the order of the sums is not fixed.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_grid_direct(long long n, long long F, long long P,
+                                                     const double* uu, const double* vv,
+                                                     const double* ww, const float* vr,
+                                                     const float* vi, const float* wt,
+                                                     Geometry g, Real* cube)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SampleWin s;
+    if (!sample_window(uu[i], vv[i], ww[i], g, s) || s.j0 > s.j1 || s.k0 > s.k1) return;
+    double sr, si;
+    sample_vis(i, F, P, vr, vi, wt, sr, si);
+    const double norm = 0.5 / LOFAR_REF_PI;
+    Real* plane = cube + 2 * (long long)s.gw * g.ny * g.nx;
+    for (int k = s.k0; k <= s.k1; k++) {
+        const double dv = (double)k + 0.5 - s.pv;
+        for (int j = s.j0; j <= s.j1; j++) {
+            const double du = (double)j + 0.5 - s.pu;
+            const double cw = norm * exp(-(du * du + dv * dv) * 0.5);
+            Real* cell = plane + 2 * ((long long)k * g.nx + j);
+            atomicAdd(cell, (Real)(sr * cw));
+            atomicAdd(cell + 1, (Real)(si * cw));
+        }
+    }
+}
+
+// One thread per sample: weight * sum over its clipped window of cw * cube (in double),
+// for every channel and polarisation (the adjoint of k_grid_direct).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_degrid_direct(long long n, long long F, long long P,
+                                                       const double* uu, const double* vv,
+                                                       const double* ww, const float* weight,
+                                                       Geometry g, const Real* cube, Real* mre,
+                                                       Real* mim)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SampleWin s;
+    if (!sample_window(uu[i], vv[i], ww[i], g, s)) return;  // rejected before the launch
+    const double norm = 0.5 / LOFAR_REF_PI;
+    const Real* plane = cube + 2 * (long long)s.gw * g.ny * g.nx;
+    double sr = 0.0, si = 0.0;
+    for (int k = s.k0; k <= s.k1; k++) {
+        const double dv = (double)k + 0.5 - s.pv;
+        for (int j = s.j0; j <= s.j1; j++) {
+            const double du = (double)j + 0.5 - s.pu;
+            const double cw = norm * exp(-(du * du + dv * dv) * 0.5);
+            const Real* cell = plane + 2 * ((long long)k * g.nx + j);
+            sr += cw * (double)cell[0];
+            si += cw * (double)cell[1];
+        }
+    }
+    for (long long p = 0; p < P; p++) {
+        const double wgt = (double)weight[i * P + p];
+        for (long long f = 0; f < F; f++) {
+            const long long o = i * F * P + f * P + p;
+            mre[o] = (Real)(wgt * sr);
+            mim[o] = (Real)(wgt * si);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host dispatch
 
 // Accumulation without accumulate: 0 = exact 64-bit fixed point (default),
@@ -809,6 +892,52 @@ static void launch_grid(int precision, const Geometry& g, const Buckets& b, cons
     }
 }
 
+hipError_t validate_samples(long long n, const double* uu, const double* vv, const double* ww,
+                            const Geometry& g, int* err, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u32_1, dim3(1), dim3(64), 0, st, (unsigned*)err);
+    if (n > 0)
+        hipLaunchKernelGGL(k_validate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, uu,
+                           vv, ww, g, err);
+    return hipGetLastError();
+}
+
+hipError_t grid_direct(int precision, long long n, long long F, long long P, const double* uu,
+                       const double* vv, const double* ww, const float* vr, const float* vi,
+                       const float* wt, const Geometry& g, void* cube, int accumulate,
+                       hipStream_t st)
+{
+    const long long words = 2LL * g.nw * g.ny * g.nx * (precision == LOFAR_FP64 ? 2 : 1);
+    if (!accumulate) {
+        hipError_t e = fill_u32((unsigned*)cube, words, 0u, st);
+        if (e != hipSuccess) return e;
+    }
+    if (n <= 0) return hipGetLastError();
+    const dim3 blocks((unsigned)((n + 255) / 256));
+    if (precision == LOFAR_FP64)
+        hipLaunchKernelGGL((k_grid_direct<double>), blocks, dim3(256), 0, st, n, F, P, uu, vv, ww,
+                           vr, vi, wt, g, (double*)cube);
+    else
+        hipLaunchKernelGGL((k_grid_direct<float>), blocks, dim3(256), 0, st, n, F, P, uu, vv, ww,
+                           vr, vi, wt, g, (float*)cube);
+    return hipGetLastError();
+}
+
+hipError_t degrid_direct(int precision, long long n, long long F, long long P, const double* uu,
+                         const double* vv, const double* ww, const float* wt, const Geometry& g,
+                         const void* cube, void* mre, void* mim, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 blocks((unsigned)((n + 255) / 256));
+    if (precision == LOFAR_FP64)
+        hipLaunchKernelGGL((k_degrid_direct<double>), blocks, dim3(256), 0, st, n, F, P, uu, vv,
+                           ww, wt, g, (const double*)cube, (double*)mre, (double*)mim);
+    else
+        hipLaunchKernelGGL((k_degrid_direct<float>), blocks, dim3(256), 0, st, n, F, P, uu, vv,
+                           ww, wt, g, (const float*)cube, (float*)mre, (float*)mim);
+    return hipGetLastError();
+}
+
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
                       void* cube, int accumulate, unsigned long long* occ, int sm_log,
                       hipStream_t st)
@@ -821,7 +950,7 @@ hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const 
         launch_grid<7>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     else if (win <= 9)
         launch_grid<9>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
-    else if (win <= 17)
+    else if (win <= kMaxTileWin)
         launch_grid<17>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     else
         return hipErrorInvalidValue;
@@ -850,7 +979,7 @@ hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, cons
         launch_degrid<7>(precision, g, b, ents, cube, weight, F, P, mre, mim, st);
     else if (win <= 9)
         launch_degrid<9>(precision, g, b, ents, cube, weight, F, P, mre, mim, st);
-    else if (win <= 17)
+    else if (win <= kMaxTileWin)
         launch_degrid<17>(precision, g, b, ents, cube, weight, F, P, mre, mim, st);
     else
         return hipErrorInvalidValue;

@@ -45,6 +45,31 @@ __device__ __forceinline__ bool sample_window(double u, double v, double w, cons
     return true;
 }
 
+// Channel/polarisation sum of one sample with the reference's weighting:
+// float product weight*vis, promoted to double, NaN real parts skipped
+// (w-stacking_omp.c:116-133).
+__device__ __forceinline__ void sample_vis(long long i, long long F, long long P, const float* vr,
+                                           const float* vi, const float* wt, double& sr,
+                                           double& si)
+{
+    sr = 0.0;
+    si = 0.0;
+    const long long vbase = i * F * P;
+    for (long long f = 0; f < F; f++) {
+        for (long long p = 0; p < P; p++) {
+            const long long ifine = vbase + f * P + p;
+            const float x = vr[ifine];
+            if (!isnan(x)) {
+                const float w = wt[i * P + p];
+                const float a = w * x;
+                const float b = w * vi[ifine];
+                sr += (double)a;
+                si += (double)b;
+            }
+        }
+    }
+}
+
 // Home cell of a sample for degridding: its truncated position clamped to the grid.
 __device__ __forceinline__ void home_cell(const Window& s, const Geometry& g, int& cu, int& cv)
 {
@@ -144,11 +169,14 @@ struct alignas(32) L2Block {
 struct Buckets {
     TileMap map;
     long long nwg, chunk;  // level-1 workgroups and samples per workgroup
+    bool global_l1;        // level-1 histograms and cursors in global memory (more
+                           // super-tiles than the LDS histograms hold)
     unsigned* count1;      // [nk1] records per super-tile
     long long* off1;       // [nk1+1] record offsets
     unsigned* count1e;     // [nk1] entries per super-tile
     long long* eoff1;      // [nk1+1] entry offsets
     unsigned* base1;       // [nwg*nk1] each workgroup's first slot in each super-tile
+                           // ([nk1] global cursors when global_l1)
     long long* offsets;    // [nbuckets+1] entries of every (w, tile) bucket, bucket order
     long long* partials;   // scan partials
     long long* total;      // host-visible total entry count (status[0])
@@ -188,6 +216,18 @@ hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const 
                       hipStream_t st);
 // fills n words with v (stream-ordered, no memset node)
 hipError_t fill_u32(unsigned* p, long long n, unsigned v, hipStream_t st);
+// Bucket-free kernels for windows wider than kMaxTileWin: validation of every sample
+// (*err set when one fails sample_window), gridding with global atomic adds (the cube is
+// zeroed first unless accumulate), and a per-sample window gather for degridding.
+hipError_t validate_samples(long long n, const double* uu, const double* vv, const double* ww,
+                            const Geometry& g, int* err, hipStream_t st);
+hipError_t grid_direct(int precision, long long n, long long F, long long P, const double* uu,
+                       const double* vv, const double* ww, const float* vr, const float* vi,
+                       const float* wt, const Geometry& g, void* cube, int accumulate,
+                       hipStream_t st);
+hipError_t degrid_direct(int precision, long long n, long long F, long long P, const double* uu,
+                         const double* vv, const double* ww, const float* wt, const Geometry& g,
+                         const void* cube, void* model_re, void* model_im, hipStream_t st);
 hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, const DEntry* entries,
                         const void* cube, const float* weight, long long F, long long P,
                         void* model_re, void* model_im, hipStream_t st);

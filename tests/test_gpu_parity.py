@@ -182,14 +182,18 @@ print("BUCKETS_OK")
 """
 
 
-@pytest.mark.parametrize("cfg", ["1,256,256", "2,1000,300", "16,300,100000", "8,1000000,8192"])
+@pytest.mark.parametrize("cfg", ["1,256,256", "2,1000,300", "16,300,100000", "8,1000000,8192",
+                                 "global"])
 def test_bucket_sort_configurations(cuda, cfg):
     """Other super-tile sides, level-1 chunk and level-2 block sizes (LOFAR_BUCKET_CFG, read once per
-    process) give the same cube; run in a child process with the setting applied."""
+    process) give the same cube; "global": level 1 with global histograms and cursors (the
+    path for more super-tiles than the LDS histograms hold, forced here with
+    LOFAR_L1_LDS_MAX=2).  Run in a child process with the setting applied."""
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    env = dict(os.environ, LOFAR_BUCKET_CFG=cfg)
+    env = dict(os.environ, LOFAR_L1_LDS_MAX="2") if cfg == "global" else \
+        dict(os.environ, LOFAR_BUCKET_CFG=cfg)
     r = subprocess.run([sys.executable, "-c", _BUCKET_SCRIPT, root], env=env, cwd=root,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "BUCKETS_OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
@@ -535,3 +539,125 @@ def test_adjoint_dot_test(cuda, precision, tol):
     print("dot test %s: %.3e" % (precision, abs(lhs - rhs) / abs(lhs)))
     assert abs(lhs - rhs) < tol * abs(lhs)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("nx,nw", [(1024, 4), (4096, 2), (8192, 1)])
+def test_transforms_and_fused_passes_are_bitwise_deterministic(cuda, nx, nw):
+    """Run-to-run determinism of every pass that uses raw buffer loads and stores or
+    direct-to-LDS loads: the plane transforms (both signs, both precisions), the fused
+    imaging path (FP64 strip-major scratch and FP32C row-major, column transform over
+    empty tiles, fused or split row pass) and the adjoint give the same bits when run
+    twice on the same input."""
+    import torch
+    from lofar_amd import imager
+    rng = np.random.default_rng(nx + nw)
+    x = rng.standard_normal((nw, nx, nx)) + 1j * rng.standard_normal((nw, nx, nx))
+    for dt in (torch.complex128, torch.complex64):
+        c = torch.from_numpy(x).to(cuda).to(dt)
+        for sign in (1, -1):
+            a = imager.fft(c, sign=sign)
+            b = imager.fft(c, sign=sign)
+            assert torch.equal(a.view(torch.float32 if dt == torch.complex64 else torch.float64),
+                               b.view(torch.float32 if dt == torch.complex64 else torch.float64))
+        del c
+    del x
+    d = synth(300000, nx, max(nw, 2), seed=nx)
+    nww = max(nw, 2)
+    for prec in ("fp64", "fp32c"):
+        outs = []
+        for _ in range(2):
+            r, i = imager.image(*dev_inputs(d, cuda), nww, 7, nx, nx, d["resolution"], d["wmin"],
+                                d["wmax"], precision=prec)
+            outs.append((H(r), H(i)))
+        assert np.array_equal(outs[0][0].view(np.uint64), outs[1][0].view(np.uint64))
+        assert np.array_equal(outs[0][1].view(np.uint64), outs[1][1].view(np.uint64))
+        img_re = torch.from_numpy(outs[0][0]).to(cuda)
+        img_im = torch.from_numpy(outs[0][1]).to(cuda)
+        ms = [imager.degrid(img_re, img_im, T(d["uu"], cuda), T(d["vv"], cuda), T(d["ww"], cuda),
+                            T(d["weight"], cuda), nww, 7, d["resolution"], d["wmin"], d["wmax"],
+                            precision=prec) for _ in range(2)]
+        assert torch.equal(ms[0][0], ms[1][0]) and torch.equal(ms[0][1], ms[1][1])
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("support", [19, 25, 40])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_wide_support_grid_matches_oracle(cuda, support, precision, tol):
+    """Supports whose window (2K+1) exceeds the tile kernels' 17 cells take the bucket-free
+    path (global atomics, as the reference's OpenMP loop); windows clipped at every edge,
+    NaN real parts, F = 2 channels; bad samples are rejected with nothing written."""
+    import torch
+    from lofar_amd import imager
+    from lofar_amd._lib import ECOORD, LofarError
+    d = random_case(40 + support, 6000, 160, 96, 3, F=2, edge=True)
+    st, want = O.wstack(3, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / 160, 1 / 3, support, 160, 96, freq_per_chan=2)
+    got = H(imager.grid(*dev_inputs(d, cuda), 3, support, 160, 96, precision=precision,
+                        freq_per_chan=2))
+    assert st == O.OK and rel(got, want) < tol
+    # accumulate mode adds to the cube
+    c = imager.grid(*dev_inputs(d, cuda), 3, support, 160, 96, precision=precision,
+                    freq_per_chan=2)
+    c2 = imager.grid(*dev_inputs(d, cuda), 3, support, 160, 96, precision=precision,
+                     freq_per_chan=2, cube=c.clone(), accumulate=True)
+    assert rel(H(c2), 2 * want) < tol
+    bad = dict(d)
+    bad["ww"] = d["ww"].copy()
+    bad["ww"][123] = 1.5
+    keep = torch.full_like(c, 5.0)
+    with pytest.raises(LofarError) as ei:
+        imager.grid(*dev_inputs(bad, cuda), 3, support, 160, 96, precision=precision,
+                    freq_per_chan=2, cube=keep)
+    assert ei.value.code == ECOORD and bool((keep == 5.0).all())
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-6)])
+def test_wide_support_image_and_adjoint(cuda, precision, tol):
+    """The fused imaging path with support 21 (bucket-free gridding into a row-major cube)
+    against the oracle, the degridder against the oracle's interpolation, and the dot test."""
+    from lofar_amd import imager
+    nx, nw, support = 256, 4, 21
+    d = synth(20000, nx, nw, seed=21)
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                   support, nx, nx, d["resolution"], d["wmin"], d["wmax"])
+    r, i = imager.image(*dev_inputs(d, cuda), nw, support, nx, nx, d["resolution"], d["wmin"],
+                        d["wmax"], precision=precision)
+    ax = H(r) + 1j * H(i)
+    assert rel(ax, want) < tol
+    rng = np.random.default_rng(22)
+    # y = Ax (normalised) plus noise, so that <y, Ax> is not a near-cancelling sum
+    y = ax / np.sqrt(np.mean(np.abs(ax) ** 2)) + (rng.standard_normal((nx, nx)) +
+                                                   1j * rng.standard_normal((nx, nx)))
+    want_m = O.degrid_image(y, nw, d["uu"], d["vv"], d["ww"], d["weight"], support,
+                            d["resolution"], d["wmin"], d["wmax"])
+    mr, mi = imager.degrid(T(y.real.copy(), cuda), T(y.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), nw, support,
+                           d["resolution"], d["wmin"], d["wmax"], precision=precision)
+    ahy = H(mr).astype(np.float64) + 1j * H(mi).astype(np.float64)
+    assert rel(ahy, want_m) < (1e-10 if precision == "fp64" else 1e-5)
+    x = d["vis_real"].astype(np.float64) + 1j * d["vis_img"].astype(np.float64)
+    lhs, rhs = np.vdot(y, ax), np.vdot(ahy, x)
+    assert abs(lhs - rhs) < (1e-12 if precision == "fp64" else 1e-6) * abs(lhs)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-6)])
+def test_more_super_tiles_than_lds_histograms(cuda, precision, tol):
+    """9000 w-planes of 64x64 cells make 9000 super-tiles, more than the 8192 the LDS
+    level-1 histograms hold: the bucket sort takes its global-histogram path and the
+    image and the degridding still match the oracle."""
+    from lofar_amd import imager
+    nx, nw = 64, 9000
+    d = synth(60000, nx, nw, seed=9000)
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
+                   nx, nx, d["resolution"], d["wmin"], d["wmax"])
+    r, i = imager.image(*dev_inputs(d, cuda), nw, 7, nx, nx, d["resolution"], d["wmin"],
+                        d["wmax"], precision=precision)
+    assert rel(H(r) + 1j * H(i), want) < tol
+    rng = np.random.default_rng(9001)
+    y = rng.standard_normal((nx, nx)) + 1j * rng.standard_normal((nx, nx))
+    want_m = O.degrid_image(y, nw, d["uu"], d["vv"], d["ww"], d["weight"], 7, d["resolution"],
+                            d["wmin"], d["wmax"])
+    mr, mi = imager.degrid(T(y.real.copy(), cuda), T(y.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), nw, 7,
+                           d["resolution"], d["wmin"], d["wmax"], precision=precision)
+    assert rel(H(mr) + 1j * H(mi), want_m) < (1e-10 if precision == "fp64" else 1e-5)
