@@ -26,7 +26,7 @@ extern "C" {
 #define LOFAR_OK 0
 #define LOFAR_EARG (-1)    /* bad scalar argument or NULL pointer */
 #define LOFAR_ECOORD (-2)  /* a (u,v,w) sample is not finite or lies outside the w-planes */
-#define LOFAR_ESIZE (-3)   /* FFT side not a power of two in [2, 8192] */
+#define LOFAR_ESIZE (-3)   /* FFT side outside [2, 8192] or with a prime factor above 13 */
 #define LOFAR_EHIP (-4)    /* HIP runtime error */
 #define LOFAR_ENOMEM (-5)  /* device allocation failed */
 #define LOFAR_ENODEV (-6)  /* no usable gfx950 device */
@@ -65,7 +65,9 @@ int test(int nnn);
 /* The per-plane transform of fftw_data() (fourier_transform.c:7-74), which has
  * no function boundary in the reference: for each of nw planes,
  * gridss = FFTW_BACKWARD(grid) / (nx*ny) over dims (n0=ny, n1=nx).  nx and ny
- * must be powers of two in [2, 8192].  grid and gridss may alias. */
+ * are in [2, 8192] with no prime factor above 13 (powers of two take the radix-8/16
+ * kernels, other sides mixed-radix 2/3/4/5/7/8/11/13 ones).  grid and gridss may
+ * alias. */
 int lofar_fft_planes(const double* grid, double* gridss, int nx, int ny, int nw);
 
 /* Status of the calling thread's last drop-in call (LOFAR_OK if it succeeded). */

@@ -244,6 +244,44 @@ const char* kStageNames[LOFAR_NSTAGES] = {
 
 bool pow2_ok(int n) { return n >= 2 && n <= 8192 && (n & (n - 1)) == 0; }
 
+// Mixed-radix transforms of every plane (sides that are not powers of two): columns
+// (with `col_scale`), then rows (with `row_scale`); out may alias in.
+int fft_generic_planes(DeviceCtx& c, Arena* ar, int precision, int sign, const void* in,
+                       void* out, int nx, int ny, int nw, double row_scale, hipStream_t st,
+                       bool rows_first = false)
+{
+    void *twx, *twy;
+    int r = twiddles_for(c, ar, nx, precision, st, &twx);
+    if (r) return r;
+    r = twiddles_for(c, ar, ny, precision, st, &twy);
+    if (r) return r;
+    auto cols = [&](const void* src) {
+        return precision == LOFAR_FP64
+                   ? fft_generic<double>(sign, src, out, twy, ny, (long long)nw * nx, nx,
+                                         (long long)nx * ny, 1, nx, rows_first ? row_scale : 1.0,
+                                         st)
+                   : fft_generic<float>(sign, src, out, twy, ny, (long long)nw * nx, nx,
+                                        (long long)nx * ny, 1, nx, rows_first ? row_scale : 1.0,
+                                        st);
+    };
+    auto rows = [&](const void* src) {
+        const long long lines = (long long)nw * ny;
+        return precision == LOFAR_FP64
+                   ? fft_generic<double>(sign, src, out, twx, nx, lines, lines, 0, nx, 1,
+                                         rows_first ? 1.0 : row_scale, st)
+                   : fft_generic<float>(sign, src, out, twx, nx, lines, lines, 0, nx, 1,
+                                        rows_first ? 1.0 : row_scale, st);
+    };
+    if (rows_first) {
+        HIPCHK(stage(c, kStRows, st, [&] { return rows(in); }));
+        HIPCHK(stage(c, kStCols, st, [&] { return cols(out); }));
+    } else {
+        HIPCHK(stage(c, kStCols, st, [&] { return cols(in); }));
+        HIPCHK(stage(c, kStRows, st, [&] { return rows(out); }));
+    }
+    return LOFAR_OK;
+}
+
 int check_grid_args(int precision, int nw, long n, long F, long P, double dx, double dw,
                     int support, int nx, int ny)
 {
@@ -463,7 +501,9 @@ int fft_impl(DeviceCtx& c, int precision, int sign, const void* in, void* out, i
     if (precision != LOFAR_FP64 && precision != LOFAR_FP32C) return fail(LOFAR_EARG);
     if (sign != 1 && sign != -1) return fail(LOFAR_EARG);
     if (nw < 1 || !in || !out) return fail(LOFAR_EARG);
-    if (!pow2_ok(nx) || !pow2_ok(ny)) return fail(LOFAR_ESIZE);
+    if (!fft_size_ok(nx) || !fft_size_ok(ny)) return fail(LOFAR_ESIZE);
+    if (!pow2_ok(nx) || !pow2_ok(ny))
+        return fft_generic_planes(c, nullptr, precision, sign, in, out, nx, ny, nw, scale, st);
     void *twx, *twy;
     int r = twiddles(c, nx, precision, &twx);
     if (r) return r;
@@ -569,7 +609,7 @@ const char* lofar_error_string(int code)
         case LOFAR_OK: return "ok";
         case LOFAR_EARG: return "bad argument";
         case LOFAR_ECOORD: return "sample not finite or outside the w-planes";
-        case LOFAR_ESIZE: return "FFT side must be a power of two in [2, 8192]";
+        case LOFAR_ESIZE: return "FFT side must be in [2, 8192] with no prime factor above 13";
         case LOFAR_EHIP: return "HIP runtime error";
         case LOFAR_ENOMEM: return "device allocation failed";
         case LOFAR_ENODEV: return "no usable gfx950 device";
@@ -691,7 +731,10 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
     // windows wider than the tile kernels take are gridded without buckets into a whole
     // row-major cube
     const bool direct = w_support >= 1 && 2 * ((w_support - 1) / 2) + 1 > kMaxTileWin;
-    if (plane_bytes < (1LL << 31) && !direct) {
+    // sides that are not powers of two: the mixed-radix transforms and the standalone
+    // phase sum over a whole row-major cube
+    const bool generic = !pow2_ok(grid_size_x) || !pow2_ok(grid_size_y);
+    if (plane_bytes < (1LL << 31) && !direct && !generic) {
         const long long ntv = (grid_size_y + kTile - 1) / kTile;
         const long long ntu = (grid_size_x + kTile - 1) / kTile;
         const long long nocc = (long long)num_w_planes * ntu * ((ntv + 63) / 64);
@@ -706,7 +749,7 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
     // contiguous block and the column transforms read 64-byte runs 1 KiB apart instead
     // of a plane row apart; FP32C keeps row-major planes (no gain measured there).
     // LOFAR_SM_LOG overrides, for measurement (-1: row-major; at most the tile width).
-    const int sm_pref = direct ? -1 : sm_log_cfg(precision);
+    const int sm_pref = (direct || generic) ? -1 : sm_log_cfg(precision);
     const int sm_log = (sm_pref >= 0 && (1 << sm_pref) <= kTile &&
                         grid_size_x % (1 << sm_pref) == 0 &&
                         (1 << sm_pref) % (precision == LOFAR_FP64 ? 4 : 8) == 0)
@@ -716,6 +759,17 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
                   ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x, grid_size_y, cube,
                   st, occ, sm_log, ar);
     if (r) return r;
+    if (generic) {
+        r = fft_generic_planes(c, ar, precision, 1, cube, cube, grid_size_x, grid_size_y,
+                               num_w_planes, 1.0 / ((double)grid_size_x * grid_size_y), st);
+        if (r) return r;
+        HIPCHK(stage(c, kStPhase, st, [&] {
+            return phase_sum(precision, cube, image_real, image_imag, grid_size_x, grid_size_y,
+                             num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax,
+                             accumulate, st);
+        }));
+        return LOFAR_OK;
+    }
     void *twx, *twy;
     r = twiddles_for(c, ar, grid_size_x, precision, st, &twx);
     if (r) return r;
@@ -768,7 +822,7 @@ int lofar_image_dev(int precision, int num_w_planes, long num_points, long freq_
                     double wmin, double wmax, void* cube, double* image_real, double* image_imag,
                     int accumulate, void* stream)
 {
-    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    if (!fft_size_ok(grid_size_x) || !fft_size_ok(grid_size_y)) return fail(LOFAR_ESIZE);
     if (!image_real || !image_imag) return fail(LOFAR_EARG);
     CallGuard cg(stream);
     if (cg.status) return cg.status;
@@ -786,7 +840,7 @@ int lofar_image_dev_static(int precision, int num_w_planes, long num_points,
                            double wmin, double wmax, void* cube, double* image_real,
                            double* image_imag, int accumulate, void* stream)
 {
-    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    if (!fft_size_ok(grid_size_x) || !fft_size_ok(grid_size_y)) return fail(LOFAR_ESIZE);
     if (!image_real || !image_imag) return fail(LOFAR_EARG);
     CallGuard cg(stream);
     if (cg.status) return cg.status;
@@ -858,7 +912,7 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
                      const double* image_real, const double* image_imag, void* cube,
                      void* model_real, void* model_imag, void* stream)
 {
-    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    if (!fft_size_ok(grid_size_x) || !fft_size_ok(grid_size_y)) return fail(LOFAR_ESIZE);
     int r = check_grid_args(precision, num_w_planes, num_points, freq_per_chan, polarizations, dx,
                             dw, w_support, grid_size_x, grid_size_y);
     if (r) return fail(r);
@@ -875,7 +929,18 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
         const PhaseParams pp =
             phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
         const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
-        if (grid_size_x >= 8192 && rows_split()) {
+        if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) {
+            // sides that are not powers of two: conjugate screen, then the mixed-radix
+            // forward transforms (rows, then columns with the 1/(nx*ny))
+            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
+                return phase_adjoint(precision, image_real, image_imag, cube, grid_size_x,
+                                     grid_size_y, num_w_planes, grid_size_x, grid_size_y,
+                                     resolution, wmin, wmax, cg.st);
+            }));
+            r = fft_generic_planes(c, nullptr, precision, -1, cube, cube, grid_size_x,
+                                   grid_size_y, num_w_planes, scale, cg.st, true);
+            if (r) return r;
+        } else if (grid_size_x >= 8192 && rows_split()) {
             // rows of 8192: the fused adjoint row pass does not fit its 1024-thread
             // workgroups' registers, so the conjugate screen (every plane written) and
             // the forward row transform (in place) run as two passes
@@ -1041,7 +1106,7 @@ int lofar_fft_planes(const double* grid, double* gridss, int nx, int ny, int nw)
 {
     t_last_error = LOFAR_OK;
     if (!grid || !gridss || nw < 1) return fail(LOFAR_EARG);
-    if (!pow2_ok(nx) || !pow2_ok(ny)) return fail(LOFAR_ESIZE);
+    if (!fft_size_ok(nx) || !fft_size_ok(ny)) return fail(LOFAR_ESIZE);
     CallGuard cg(nullptr);
     if (cg.status) return cg.status;
     DeviceCtx& c = *cg.c;

@@ -650,13 +650,181 @@ hipError_t rows_phase_adj(void* cube, const void* tw, int nx, int ny, int nw,
 #undef CALL_
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-radix transforms of any line length whose prime factors are at most 13 (the
+// sides fftw_data() plans that are not powers of two, fourier_transform.c:25-27).
+// One 256-thread workgroup per line; the line sits in LDS and each Stockham stage of
+// radix P reads all of a thread's butterfly inputs into registers, then (after a
+// barrier) writes the outputs in place: butterfly j (k = j mod Ns, Ns the product of
+// the radices before this stage) takes x[j + q*N/P], q < P, times exp(+-2 pi i q k /
+// (Ns P)), DFT over q, and stores y[(j/Ns)*Ns*P + k + r*Ns].  Twiddles and the
+// radix-P roots come from the N-entry table exp(+2 pi i m/N).
+
+constexpr int kGenT = 256;       // threads per line
+constexpr int kGenMaxN = 8192;   // longest line
+constexpr int kGenMaxStages = 16;
+
+struct GenPlan {
+    int n, nstages;
+    int radix[kGenMaxStages];
+};
+
+template <int P, int SIGN, typename T2>
+__device__ __forceinline__ void gen_stage(T2* buf, int n, int ns, int t,
+                                          const T2* __restrict__ tw)
+{
+    constexpr int NB = (kGenMaxN / P + kGenT - 1) / kGenT;  // butterflies per thread, at most
+    const int nb = n / P;
+    const int step = n / (ns * P);  // table stride of exp(2 pi i / (Ns P))
+    T2 a[NB][P];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int j = t + i * kGenT;
+        if (j < nb) {
+#pragma unroll
+            for (int q = 0; q < P; q++) a[i][q] = buf[j + q * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int j = t + i * kGenT;
+        if (j < nb) {
+            const int k = j % ns;
+            if (ns > 1) {
+#pragma unroll
+                for (int q = 1; q < P; q++) {
+                    const T2 w = tw[(long long)q * k * step % n];
+                    a[i][q] = c_mul_tw<SIGN>(a[i][q], w);
+                }
+            }
+            T2 b[P];
+            if constexpr (P == 2 || P == 4 || P == 8) {
+#pragma unroll
+                for (int q = 0; q < P; q++) b[q] = a[i][q];
+                dft<P, SIGN>(b);
+            } else {
+                const int root = n / P;  // table stride of exp(2 pi i / P)
+#pragma unroll
+                for (int r = 0; r < P; r++) {
+                    T2 acc = a[i][0];
+#pragma unroll
+                    for (int q = 1; q < P; q++) {
+                        const T2 w = tw[(long long)((q * r) % P) * root];
+                        const T2 x = c_mul_tw<SIGN>(a[i][q], w);
+                        acc.x += x.x;
+                        acc.y += x.y;
+                    }
+                    b[r] = acc;
+                }
+            }
+            const int base = (j / ns) * ns * P + k;
+#pragma unroll
+            for (int r = 0; r < P; r++) buf[base + r * ns] = b[r];
+        }
+    }
+    __syncthreads();
+}
+
+// lines: line l (l < nlines) starts at in/out + (l / per) * plane + (l % per) * lstride and
+// its elements are estride apart (rows: per = lines, lstride = N, estride = 1; columns of
+// [nw][ny][nx] planes: per = nx, lstride = 1, estride = nx, plane = nx*ny)
+template <typename Real, int SIGN>
+__global__ __launch_bounds__(kGenT) void k_fft_generic(const C2<Real>* in, C2<Real>* out,
+                                                       const C2<Real>* tw, GenPlan plan,
+                                                       long long per, long long plane,
+                                                       long long lstride, long long estride,
+                                                       Real scale)
+{
+    using T2 = C2<Real>;
+    __shared__ T2 buf[kGenMaxN];
+    const long long l = blockIdx.x;
+    const long long base = (l / per) * plane + (l % per) * lstride;
+    const int n = plan.n, t = threadIdx.x;
+    for (int e = t; e < n; e += kGenT) buf[e] = in[base + e * estride];
+    __syncthreads();
+    int ns = 1;
+    for (int s = 0; s < plan.nstages; s++) {
+        const int p = plan.radix[s];
+        switch (p) {
+            case 2: gen_stage<2, SIGN>(buf, n, ns, t, tw); break;
+            case 3: gen_stage<3, SIGN>(buf, n, ns, t, tw); break;
+            case 4: gen_stage<4, SIGN>(buf, n, ns, t, tw); break;
+            case 5: gen_stage<5, SIGN>(buf, n, ns, t, tw); break;
+            case 7: gen_stage<7, SIGN>(buf, n, ns, t, tw); break;
+            case 8: gen_stage<8, SIGN>(buf, n, ns, t, tw); break;
+            case 11: gen_stage<11, SIGN>(buf, n, ns, t, tw); break;
+            default: gen_stage<13, SIGN>(buf, n, ns, t, tw); break;
+        }
+        ns *= p;
+    }
+    for (int e = t; e < n; e += kGenT) {
+        T2 x = buf[e];
+        x.x *= scale;
+        x.y *= scale;
+        out[base + e * estride] = x;
+    }
+}
+
+// radices of n (8s, then 4, 2, 3, 5, 7, 11, 13); false when another prime divides n
+bool generic_plan(int n, GenPlan& p)
+{
+    if (n < 2 || n > kGenMaxN) return false;
+    p.n = n;
+    p.nstages = 0;
+    int m = n;
+    const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    for (int r : order) {
+        while (m % r == 0) {
+            if (p.nstages == kGenMaxStages) return false;
+            p.radix[p.nstages++] = r;
+            m /= r;
+        }
+    }
+    return m == 1;
+}
+
+bool fft_size_ok(int n)
+{
+    if (n >= 2 && n <= 8192 && (n & (n - 1)) == 0) return true;
+    GenPlan p;
+    return generic_plan(n, p);
+}
+
+template <typename Real>
+hipError_t fft_generic(int sign, const void* in, void* out, const void* tw, int n, long long lines,
+                       long long per, long long plane, long long lstride, long long estride,
+                       double scale, hipStream_t st)
+{
+    GenPlan p;
+    if (!generic_plan(n, p)) return hipErrorInvalidValue;
+    if (lines <= 0) return hipSuccess;
+    using T2 = C2<Real>;
+    if (sign > 0)
+        hipLaunchKernelGGL((k_fft_generic<Real, 1>), dim3((unsigned)lines), dim3(kGenT), 0, st,
+                           (const T2*)in, (T2*)out, (const T2*)tw, p, per, plane, lstride,
+                           estride, (Real)scale);
+    else
+        hipLaunchKernelGGL((k_fft_generic<Real, -1>), dim3((unsigned)lines), dim3(kGenT), 0, st,
+                           (const T2*)in, (T2*)out, (const T2*)tw, p, per, plane, lstride,
+                           estride, (Real)scale);
+    return hipGetLastError();
+}
+
+template hipError_t fft_generic<double>(int, const void*, void*, const void*, int, long long,
+                                        long long, long long, long long, long long, double,
+                                        hipStream_t);
+template hipError_t fft_generic<float>(int, const void*, void*, const void*, int, long long,
+                                       long long, long long, long long, long long, double,
+                                       hipStream_t);
+
 template <typename Real>
 __global__ __launch_bounds__(256) void k_twiddles(C2<Real>* tw, int n)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
     double sn, cs;
-    sincospi(2.0 * (double)m / (double)n, &sn, &cs);  // m/n exact: n is a power of two
+    sincospi(2.0 * (double)m / (double)n, &sn, &cs);
     tw[m] = C2<Real>{(Real)cs, (Real)sn};
 }
 

@@ -455,6 +455,16 @@ PhaseParams phase_params(int nw, int xtot, int ytot, double resolution, double w
 // rounded to float for LOFAR_FP32C): the allocation-free entry point's twiddle tables
 hipError_t fill_twiddles(int precision, int n, void* tw, hipStream_t st);
 
+// Mixed-radix line transforms for sides that are not powers of two (prime factors at most
+// 13, side at most 8192): line l of `lines` starts at (l / per) * plane + (l % per) *
+// lstride, its elements estride apart; out may alias in.  fft_size_ok: a side either the
+// power-of-two kernels or these take.
+bool fft_size_ok(int n);
+template <typename Real>
+hipError_t fft_generic(int sign, const void* in, void* out, const void* tw, int n, long long lines,
+                       long long per, long long plane, long long lstride, long long estride,
+                       double scale, hipStream_t st);
+
 // host dispatch (fft.hip); all return hipErrorInvalidValue for unsupported sizes
 template <typename Real>
 hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx, long long lines,

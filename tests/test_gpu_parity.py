@@ -277,7 +277,8 @@ def test_fft_8192_single_plane(cuda):
     assert rel(H(imager.fft(T(x.astype(np.complex128), cuda))), want) < 1e-13
 
 
-def test_fft_in_place_and_rejects_non_pow2(cuda):
+def test_fft_in_place_and_rejects_unsupported_sides(cuda):
+    """In-place transforms; a side with a prime factor above 13 (17, 8191) is rejected."""
     from lofar_amd import imager
     from lofar_amd._lib import ESIZE, LofarError
     rng = np.random.default_rng(3)
@@ -285,9 +286,58 @@ def test_fft_in_place_and_rejects_non_pow2(cuda):
     t = T(x, cuda)
     imager.fft(t, out=t)
     assert rel(H(t), np.fft.ifft2(x, axes=(1, 2))) < 1e-13
-    with pytest.raises(LofarError) as e:
-        imager.fft(T(np.zeros((1, 12, 16), complex), cuda))
-    assert e.value.code == ESIZE
+    for shape in ((1, 12, 17), (1, 8191, 4)):
+        with pytest.raises(LofarError) as e:
+            imager.fft(T(np.zeros(shape, complex), cuda))
+        assert e.value.code == ESIZE
+
+
+@pytest.mark.parametrize("nx,ny,nw", [(12, 16, 2), (96, 60, 3), (300, 300, 2), (1000, 1000, 1),
+                                      (6, 10, 1), (3000, 8, 1), (8, 3000, 1), (7, 11, 2),
+                                      (8000, 4, 1), (13, 26, 1), (6000, 20, 1)])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_fft_mixed_radix_matches_numpy(cuda, nx, ny, nw, sign):
+    """Sides that are not powers of two (fftw_data plans any size, fourier_transform.c:25-27)
+    take the mixed-radix transforms: FP64 within 1e-13 and FP32 within 2e-6 of numpy."""
+    from lofar_amd import imager
+    rng = np.random.default_rng(nx * 7 + ny)
+    x = rng.standard_normal((nw, ny, nx)) + 1j * rng.standard_normal((nw, ny, nx))
+    want = np.fft.ifft2(x, axes=(1, 2)) if sign > 0 else np.fft.fft2(x, axes=(1, 2)) / (nx * ny)
+    got64 = H(imager.fft(T(x, cuda), sign=sign))
+    assert rel(got64, want) < 1e-13
+    got32 = H(imager.fft(T(x.astype(np.complex64), cuda), sign=sign))
+    assert rel(got32, want) < 2e-6
+
+
+@pytest.mark.parametrize("nx,ny,nw", [(300, 300, 4), (96, 160, 3)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-6)])
+def test_mixed_radix_image_and_adjoint_match_oracle(cuda, nx, ny, nw, precision, tol):
+    """Imaging and its adjoint on grids whose sides are not powers of two (and non-square)
+    against the oracle, and the drop-in fftw_data on such a cube."""
+    from lofar_amd import imager
+    d = random_case(nx + ny, 30000, nx, ny, nw, edge=False, nan=0.01)
+    res, wmin, wmax = 1.0 / (2 * max(nx, ny)), -50.0, 70.0
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
+                   nx, ny, res, wmin, wmax)
+    r, i = imager.image(*dev_inputs(d, cuda), nw, 7, nx, ny, res, wmin, wmax,
+                        precision=precision)
+    assert rel_nan(H(r) + 1j * H(i), want) < tol
+    rng = np.random.default_rng(nx)
+    y = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    want_m = O.degrid_image(y, nw, d["uu"], d["vv"], d["ww"], d["weight"], 7, res, wmin, wmax)
+    mr, mi = imager.degrid(T(y.real.copy(), cuda), T(y.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), nw, 7, res,
+                           wmin, wmax, precision=precision)
+    assert rel(H(mr) + 1j * H(mi), want_m) < (1e-10 if precision == "fp64" else 1e-5)
+    if precision == "fp64":
+        grid = np.zeros(2 * nw * nx * ny)
+        imager.wstack(nw, len(d["uu"]), 1, 1, d["uu"], d["vv"], d["ww"], d["vis_real"],
+                      d["vis_img"], d["weight"], 1 / nx, 1 / nw, 7, nx, ny, grid)
+        gridss = np.zeros_like(grid)
+        imager.fftw_data(grid, gridss, nx, ny, nw)
+        cube = grid.view(np.complex128).reshape(nw, ny, nx)
+        assert rel(gridss.view(np.complex128).reshape(nw, ny, nx),
+                   np.fft.ifft2(cube, axes=(1, 2))) < 1e-13
 
 
 def test_fftw_data_dropin(cuda):
