@@ -289,6 +289,9 @@ def bench_stream(dev, n_base=16 * 1024 * 1024, nchan=16, nx=8192, nw=128, suppor
     torch.cuda.synchronize()
     ms_cyc = (time.perf_counter() - t0) * 1e3 / cycles
     h = [float(v) for v in hist]
+    cyc_stages = profile_stages(
+        lambda: major_cycles(op, s["vis_real"], s["vis_img"], s["weight"], x0, x0.clone(),
+                             cycles=1, step=step), 1)
     print("bench: stream major cycle %.1f ms, chi2 %.4e -> %.4e" % (ms_cyc, h[0], h[-1]),
           file=sys.stderr, flush=True)
     del op, cube, x0
@@ -314,6 +317,7 @@ def bench_stream(dev, n_base=16 * 1024 * 1024, nchan=16, nx=8192, nw=128, suppor
         "image_stages_ms": {k: round(v[0], 3) for k, v in img_stages.items()},
         "image_algorithmic_GBs": (36 * n + 4 * cube_bytes + 16 * nx * nx) / (ms_img * 1e-3) / 1e9,
         "major_cycle_ms": ms_cyc, "cycles": cycles, "chi2": h,
+        "major_cycle_stages_ms": {k: round(v[0] * v[1], 3) for k, v in cyc_stages.items()},
         "image_rel_rms_err_vs_fp64": rms, "image_rel_max_err_vs_fp64": mx,
     }
     del s, vis, re32, im32, r64, i64

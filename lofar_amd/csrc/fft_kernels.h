@@ -448,6 +448,32 @@ __device__ __forceinline__ void sincos_w(double x, double* sn, double* cs)
     *cs = ((q + 1) & 2) ? -c_ : c_;
 }
 
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// w-phase factor of one pixel for the plane recurrence: z0 = dwnorm*fac*exp(i*phi_0)
+// and step = exp(i*(phi_1-phi_0)), with phi_w = 2*PI*(wmin + w*dw)*(fac-1)
+// (phase_correction.cu:160,182-183,193-195), so plane w's factor is z0*step^w.
+// Without the phase (a single plane) both are 1.  FP64 throughout.
+__device__ __forceinline__ void phase_rotation(const PhaseParams& pp, double fac, double2& z0,
+                                               double2& step)
+{
+    if (!pp.phase_on) {
+        z0 = double2{1.0, 0.0};
+        step = double2{1.0, 0.0};
+        return;
+    }
+    const double n1 = fac - 1.0;
+    double sn, cs;
+    sincos_w(2.0 * LOFAR_REF_PI * pp.wmin * n1, &sn, &cs);
+    const double f = pp.dwnorm * fac;
+    z0 = double2{cs * f, sn * f};
+    sincos_w(2.0 * LOFAR_REF_PI * pp.dw * n1, &sn, &cs);
+    step = double2{cs, sn};
+}
+
 // phase constants of phase_correction.cu:92-94 (phase.hip)
 PhaseParams phase_params(int nw, int xtot, int ytot, double resolution, double wmin, double wmax);
 

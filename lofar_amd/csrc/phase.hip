@@ -8,7 +8,10 @@ namespace lofar {
 
 // phase_correction.cu:151-210 (PHASE_ON): image += sum_w plane_w * exp(i*phi_w) *
 // dwnorm * sqrt(1-r^2), phi_w = 2*PI*(wmin + w*dw)*(sqrt(1-r^2)-1); plain sum
-// when there is a single plane.
+// when there is a single plane.  FP64 cubes (the phase_correction() drop-in) evaluate
+// every plane's factor directly; complex-float cubes (the 8192-wide split row passes,
+// where the direct FP64 sincos per pixel and plane would bound the pass) advance it by
+// the FP64 rotation recurrence of the fused row pass (phase_rotation).
 template <typename Real>
 __global__ __launch_bounds__(256) void k_phase_sum(const typename cplx<Real>::type* cube,
                                                    double* img_re, double* img_im, int xaxis,
@@ -28,6 +31,17 @@ __global__ __launch_bounds__(256) void k_phase_sum(const typename cplx<Real>::ty
         fac = sqrt(1 - radius2);
     }
     double sr = 0.0, si = 0.0;
+    if constexpr (sizeof(Real) == 4) {
+        double2 z, stp;
+        phase_rotation(pp, fac, z, stp);
+        for (int iw = 0; iw < nw; iw++) {
+            const T2 x = cube[(long long)iw * npix + pix];
+            const double p = (double)x.x, q = (double)x.y;
+            sr = fma(p, z.x, fma(-q, z.y, sr));
+            si = fma(p, z.y, fma(q, z.x, si));
+            z = cmul(z, stp);
+        }
+    } else
     for (int iw = 0; iw < nw; iw++) {
         const T2 x = cube[(long long)iw * npix + pix];
         const double p = (double)x.x, q = (double)x.y;
@@ -70,6 +84,17 @@ __global__ __launch_bounds__(256) void k_phase_adjoint(const double* img_re, con
         fac = sqrt(1 - radius2);
     }
     const double a = img_re[pix], b = img_im[pix];
+    if constexpr (sizeof(Real) == 4) {
+        double2 z, stp;
+        phase_rotation(pp, fac, z, stp);
+        for (int iw = 0; iw < nw; iw++) {
+            // image * conj(z)
+            cube[(long long)iw * npix + pix] =
+                T2{(Real)(a * z.x + b * z.y), (Real)(b * z.x - a * z.y)};
+            z = cmul(z, stp);
+        }
+        return;
+    }
     for (int iw = 0; iw < nw; iw++) {
         double xr = a, xi = b;
         if (pp.phase_on) {
