@@ -249,6 +249,13 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
 // into the grid).
 constexpr int kFixedBits = 51;
 
+// Largest |vis| component of a tile over the mean of its other components above which the
+// tile is summed in floating point instead of fixed point: the fixed-point quantum is
+// ~2^-(58-log2(entries)) of the largest contribution in FP64 mode (2^-(47-...) in FP32C),
+// so below this ratio a faint cell's relative error stays under ~4e-12 (FP64) and ~1e-8
+// (FP32C, under the float rounding of the cube), and a bright outlier takes the double sums.
+constexpr double kHdrRatio = 1024.0;
+
 // Round |y| < 2^51 to the nearest integer as a two's-complement 64-bit value:
 // adding 1.5*2^52 leaves round(y) in the low 52 bits of the sum's encoding,
 // whose low word is 0 for the constant itself, so only the high word needs the
@@ -303,6 +310,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
 {
     __shared__ __align__(16) unsigned long long acc[2 * kPlane];
     __shared__ float wred[NT / 64];
+    __shared__ double wsum[NT / 64];
     __shared__ int wfin[NT / 64];
     const long long bkt = blockIdx.x;
     long long w;
@@ -339,27 +347,37 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     for (int c = tid; c < kPlane; c += NT)  // 2*kPlane words, 16 bytes a store
         ((ulonglong2*)acc)[c] = ulonglong2{0ull, 0ull};  // also 0.0 as double
     float m = fmaxf(fabsf(en0.vr), fabsf(en0.vi));
+    double sa = (double)fabsf(en0.vr) + (double)fabsf(en0.vi);
     bool finite = isfinite(en0.vr) && isfinite(en0.vi);
     for (long long e = beg + NT + tid; e < end; e += NT) {
         const Entry32 en = entries[e];
         finite = finite && isfinite(en.vr) && isfinite(en.vi);
         m = fmaxf(m, fmaxf(fabsf(en.vr), fabsf(en.vi)));
+        sa += (double)fabsf(en.vr) + (double)fabsf(en.vi);
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, off));
+        sa += __shfl_xor(sa, off);
+    }
     const bool wave_finite = __all(finite);
     if (lane == 0) {
         wred[wave] = m;
+        wsum[wave] = sa;
         wfin[wave] = wave_finite ? 1 : 0;
     }
     __syncthreads();
     m = wred[0];
+    sa = wsum[0];
     bool all_finite = wfin[0] != 0;
 #pragma unroll
     for (int i = 1; i < NT / 64; i++) {
         m = fmaxf(m, wred[i]);
+        sa += wsum[i];
         all_finite = all_finite && wfin[i] != 0;
     }
-    if (!all_finite) {
+    // a tile whose largest |vis| is far above its mean (a bright outlier) takes the
+    // floating-point path: the fixed-point quantum follows the largest contribution
+    if (!all_finite || (double)m > kHdrRatio * (sa - (double)m) / (double)(2 * n - 1)) {
         // floating-point accumulation (ds_add_f64), as k_grid_tiles<WIN, false>
         double* dacc = (double*)acc;
         for (long long e = beg + tid; e < end; e += NT) {
@@ -562,6 +580,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     constexpr int kPad = ((WIN - 1) * (kTile + 1) + 1) / 2 * 2;
     __shared__ __align__(16) unsigned long long acc[2 * kPlane + kPad];
     __shared__ double wred[NT / 64];
+    __shared__ double wsum[NT / 64];
     __shared__ int wfin[NT / 64];
     const long long bkt = blockIdx.x;
     long long w;
@@ -603,6 +622,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     for (int c = tid; c < kPlane; c += NT)  // 2*kPlane words, 16 bytes a store
         ((ulonglong2*)acc)[c] = ulonglong2{0ull, 0ull};  // also 0.0 as double
     double m = fmax(fabs(en0.vr), fabs(en0.vi));
+    double sa = fabs(en0.vr) + fabs(en0.vi);
     bool finite = isfinite(en0.vr) && isfinite(en0.vi);
     for (long long q0 = NT; q0 < n; q0 += NT) {
         const long long j = slot(q0);
@@ -610,23 +630,31 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
             const Entry64 en = entries[beg + j];
             finite = finite && isfinite(en.vr) && isfinite(en.vi);
             m = fmax(m, fmax(fabs(en.vr), fabs(en.vi)));
+            sa += fabs(en.vr) + fabs(en.vi);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmax(m, __shfl_xor(m, off));
+        sa += __shfl_xor(sa, off);
+    }
     const bool wave_finite = __all(finite);
     if (lane == 0) {
         wred[wave] = m;
+        wsum[wave] = sa;
         wfin[wave] = wave_finite ? 1 : 0;
     }
     __syncthreads();
     m = wred[0];
+    sa = wsum[0];
     bool all_finite = wfin[0] != 0;
 #pragma unroll
     for (int i = 1; i < NT / 64; i++) {
         m = fmax(m, wred[i]);
+        sa += wsum[i];
         all_finite = all_finite && wfin[i] != 0;
     }
-    if (!all_finite) {
+    // a bright outlier (largest |vis| far above the tile's mean): floating-point path
+    if (!all_finite || m > kHdrRatio * (sa - m) / (double)(2 * n - 1)) {
         double* dacc = (double*)acc;
         if (j0 >= 0) accumulate_f64<WIN>(dacc, g, u0, v0, norm, en0);
         for (long long q0 = NT; q0 < n; q0 += NT) {

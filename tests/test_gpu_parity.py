@@ -711,3 +711,30 @@ def test_more_super_tiles_than_lds_histograms(cuda, precision, tol):
                            T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), nw, 7,
                            d["resolution"], d["wmin"], d["wmax"], precision=precision)
     assert rel(H(mr) + 1j * H(mi), want_m) < (1e-10 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_grid_bright_outlier_keeps_faint_cells_precise(cuda, precision, tol):
+    """One visibility 1e6 times brighter than the rest: the fixed-point quantum of its tile
+    would follow the outlier (and cost the faint cells ~1e-8 relative in FP64 mode), so such
+    a tile is summed in floating point; the cells outside the outlier's window stay within
+    the mode's tolerance of the oracle."""
+    from lofar_amd import imager
+    n, nx, nw = 20000, 256, 2
+    d = random_case(50, n, nx, nx, nw, edge=False, nan=0)
+    i0 = 777
+    d["vis_real"] = d["vis_real"].copy()
+    d["vis_real"][i0] = 1.0e6
+    got = H(imager.grid(*dev_inputs(d, cuda), nw, 7, nx, nx, precision=precision))
+    st, want = O.wstack(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / nx, 1 / nw, 7, nx, nx)
+    assert st == O.OK
+    gu, gv, gw = int(d["uu"][i0] * nx), int(d["vv"][i0] * nx), int(d["ww"][i0] * nw)
+    faint = np.ones(want.shape, bool)
+    faint[gw, max(gv - 3, 0):gv + 4, max(gu - 3, 0):gu + 4] = False
+    # the faint cells of the outlier's own tile, and all of them
+    t = np.zeros(want.shape, bool)
+    t[gw, (gv // 64) * 64:(gv // 64) * 64 + 64, (gu // 64) * 64:(gu // 64) * 64 + 64] = True
+    assert rel(got[faint & t], want[faint & t]) < tol
+    assert rel(got[faint], want[faint]) < tol
+    assert rel(got, want) < tol
