@@ -238,10 +238,13 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
     }
 }
 
+// wt1: the samples' weights when there is one polarisation (each entry then carries its
+// sample's weight, so the degridder reads no weight array in the caller's order), else null.
 template <bool GLOBAL>
 __global__ __launch_bounds__(256) void k_l1_scatter_home(long long n, const double* uu,
                                                          const double* vv, const double* ww,
-                                                         Geometry g, TileMap tm, long long chunk,
+                                                         const float* wt1, Geometry g,
+                                                         TileMap tm, long long chunk,
                                                          const long long* off1,
                                                          unsigned* base1, DEntry* rec)
 {
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(256) void k_l1_scatter_home(long long n, const doub
         e.pu = s.pu;
         e.pv = s.pv;
         e.idx = i;
-        e.pad = 0;
+        e.pad = wt1 ? (long long)__float_as_uint(wt1[i]) : 0;
         rec[off1[k] + atomicAdd(&cur[k], 1u)] = e;
     }
 }
@@ -560,18 +563,18 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
 }
 
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
-                            const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
-                            hipStream_t st)
+                            const float* wt1, const Geometry& g, const Buckets& b, DEntry* rec,
+                            DEntry* entries, hipStream_t st)
 {
     if (n > 0 && b.global_l1) {
         hipError_t e = fill_u32(b.base1, b.map.nk1, 0u, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_l1_scatter_home<true>, dim3((unsigned)b.nwg), dim3(256), 0, st, n,
-                           uu, vv, ww, g, b.map, b.chunk, b.off1, b.base1, rec);
+                           uu, vv, ww, wt1, g, b.map, b.chunk, b.off1, b.base1, rec);
     } else if (n > 0) {
         hipLaunchKernelGGL(k_l1_scatter_home<false>, dim3((unsigned)b.nwg), dim3(256),
-                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, uu, vv, ww, g, b.map,
-                           b.chunk, b.off1, b.base1, rec);
+                           (size_t)b.map.nk1 * sizeof(unsigned), st, n, uu, vv, ww, wt1, g,
+                           b.map, b.chunk, b.off1, b.base1, rec);
     }
     return level2<kOutHome>(g, b, (const DEntry*)rec, entries, st);
 }

@@ -1085,7 +1085,8 @@ int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long
     DEntry* ents = (DEntry*)(eb + ebytes);
     level2_layout(b, total, eb + 2 * ebytes);
     HIPCHK(stage(c, kStScatter, cg.st, [&] {
-        return bucket_sort_home(num_points, uu, vv, ww, g, b, rec, ents, cg.st);
+        return bucket_sort_home(num_points, uu, vv, ww, polarizations == 1 ? weight : nullptr, g,
+                                b, rec, ents, cg.st);
     }));
     HIPCHK(stage(c, kStDegrid, cg.st, [&] {
         return degrid_tiles(precision, g, b, ents, cube, weight, freq_per_chan, polarizations,

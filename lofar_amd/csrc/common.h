@@ -59,7 +59,8 @@ __host__ __device__ inline float frac_of(unsigned w)
     return (float)(w & 0xffffffu) * 0x1p-23f - 1.0f;
 }
 
-// Degridding entry: position and the sample's index in the caller's order.
+// Degridding entry: position, the sample's index in the caller's order and (with one
+// polarisation) its weight's bits in `pad`.
 struct alignas(32) DEntry {
     double pu, pv;
     long long idx;

@@ -773,7 +773,8 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
         }
         const long long i = en.idx;
         for (long long p = 0; p < P; p++) {
-            const Real wgt = (Real)weight[i * P + p];
+            // one polarisation: the weight rides in the entry (no read in caller order)
+            const Real wgt = (Real)(P == 1 ? __uint_as_float((unsigned)en.pad) : weight[i * P + p]);
             for (long long f = 0; f < F; f++) {
                 const long long o = i * F * P + f * P + p;
                 ((Real*)mre)[o] = wgt * sr;

@@ -203,9 +203,10 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
                             const double* uu, const double* vv, const double* ww, const float* vr,
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
                             Entry64* rec, void* entries, hipStream_t st);
+// wt1 (one polarisation): the samples' weights, stored in the entries' spare word
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
-                            const Geometry& g, const Buckets& b, DEntry* rec, DEntry* entries,
-                            hipStream_t st);
+                            const float* wt1, const Geometry& g, const Buckets& b, DEntry* rec,
+                            DEntry* entries, hipStream_t st);
 // occ (optional, nw*ntu*ceil(ntv/64) words, zeroed by the caller): bit tv%64 of
 // word (w, tu, tv/64) is set for every tile holding samples; empty tiles are then
 // left unwritten (for a consumer that reads them as zeros).
