@@ -150,6 +150,43 @@ TRAFFIC_SOURCE = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_S
                   "profiles; not measured in this run")
 
 
+def bench_fp32c(dev, a, fp64_value, work, names, nx=4096, nw=32, support=7):
+    """FP32C imaging of 64M samples on the headline grid; saves the samples and the image
+    for the cpu_baseline leg when `work` is given."""
+    import torch
+    from lofar_amd import imager
+    from lofar_amd.synth import make_obs
+    ny = nx
+    n3 = 64 * 1024 * 1024
+    d3 = make_obs(n3, nx, nw, seed=2025, backend="torch", device=dev)
+    vis3 = [d3[k] for k in FIELDS]
+    cube32 = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
+    re32 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+    im32 = torch.empty_like(re32)
+
+    def step32():
+        imager.image(*vis3, nw, support, nx, ny, d3["resolution"], d3["wmin"], d3["wmax"],
+                     precision="fp32c", cube=cube32, image_real=re32, image_imag=im32)
+
+    ms32 = time_steps(step32, a.steps, a.warmup)
+    prof32 = profile_stages(step32, 3)
+    mvis32 = n3 / (ms32 * 1e-3) / 1e6
+    kb = per_kernel_bytes(n3, nx, ny, nw, 8)
+    out = {"workload": "64M samples, 4096x4096, 32 w-planes, support 7, complex64 cube",
+           "value": mvis32, "unit": "Mvis/s", "ms_per_step": ms32,
+           "speedup_vs_fp64_mode": (mvis32 / fp64_value) if fp64_value else None,
+           "end_to_end_algorithmic_GBs": kb["total"] / (ms32 * 1e-3) / 1e9,
+           "stages_ms": {k: round(v[0], 4) for k, v in prof32.items()},
+           "gridder_lds_atomics": lds_atomics(n3, support, prof32)}
+    print("bench: fp32c %.3f ms/step %.1f Mvis/s" % (ms32, mvis32), file=sys.stderr, flush=True)
+    if work is not None:
+        save_workload(work, "fp32c", d3, nx, nw, support, re32, im32)
+        names.append("fp32c")
+    del d3, vis3, cube32, re32, im32
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_adjoint(d, vis, cubeA, n2, nx, ny, nw, support, a):
     """Degridding a 4096^2 image to the 16M samples (FP32C), and the dot test
     <y, A x> = <A^H y, x> on that workload."""
@@ -341,6 +378,10 @@ def main():
                          "baseline samples x 16 channels, 120-168 MHz) streamed to the device "
                          "in 32M-sample chunks, 8192x8192 grid, 128 w-planes, support 9, FP32C, "
                          "5 major cycles; one JSON line of its own")
+    ap.add_argument("--only", choices=("fp32c", "adjoint", "chi2", "dropin"),
+                    help="only this workload (one JSON object of its own; for profiling one "
+                         "workload at a time): fp32c = 64M samples FP32C imaging; adjoint, "
+                         "chi2 and dropin on the 16M-sample headline samples")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 launches), the "
                          "known-size reference for scaling HBM byte counters")
@@ -371,9 +412,22 @@ def main():
     nx = ny = 4096
     nw, support = 32, 7
     n2 = 16 * 1024 * 1024
+    if a.only == "fp32c":
+        print(json.dumps(bench_fp32c(dev, a, None, None, None)), flush=True)
+        return
     print("bench: generating %d samples (%dx%d, %d planes)" % (n2, nx, ny, nw), file=sys.stderr, flush=True)
     d = make_obs(n2, nx, nw, seed=2024, backend="torch", device=dev)
     vis = [d[k] for k in FIELDS]
+    if a.only:
+        if a.only == "adjoint":
+            cubeA = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
+            res = bench_adjoint(d, vis, cubeA, n2, nx, ny, nw, support, a)
+        elif a.only == "chi2":
+            res = bench_chi2(d, n2, nx, ny, nw, support)
+        else:
+            res = bench_dropin(d, n2, nx, ny, nw, support)
+        print(json.dumps(res), flush=True)
+        return
     cube64 = torch.empty((nw, ny, nx), dtype=torch.complex128, device=dev)
     re64 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
     im64 = torch.empty_like(re64)
@@ -406,35 +460,9 @@ def main():
 
     extra = {}
     if not a.no_extra:
-        # FP32C on 64M samples, same grid
         del cube64
         torch.cuda.empty_cache()
-        n3 = 64 * 1024 * 1024
-        d3 = make_obs(n3, nx, nw, seed=2025, backend="torch", device=dev)
-        vis3 = [d3[k] for k in FIELDS]
-        cube32 = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
-        re32 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
-        im32 = torch.empty_like(re32)
-
-        def step32():
-            imager.image(*vis3, nw, support, nx, ny, d3["resolution"], d3["wmin"], d3["wmax"],
-                         precision="fp32c", cube=cube32, image_real=re32, image_imag=im32)
-
-        ms32 = time_steps(step32, a.steps, a.warmup)
-        prof32 = profile_stages(step32, 3)
-        mvis32 = n3 / (ms32 * 1e-3) / 1e6
-        extra["fp32c"] = {"workload": "64M samples, 4096x4096, 32 w-planes, support 7, complex64 cube",
-                          "value": mvis32, "unit": "Mvis/s", "ms_per_step": ms32,
-                          "speedup_vs_fp64_mode": mvis32 / value,
-                          "stages_ms": {k: round(v[0], 4) for k, v in prof32.items()},
-                          "gridder_lds_atomics": lds_atomics(n3, support, prof32)}
-        print("bench: fp32c %.3f ms/step %.1f Mvis/s" % (ms32, mvis32), file=sys.stderr,
-              flush=True)
-        if not a.no_cpu:
-            save_workload(work, "fp32c", d3, nx, nw, support, re32, im32)
-            names.append("fp32c")
-        del d3, vis3, cube32, re32, im32
-        torch.cuda.empty_cache()
+        extra["fp32c"] = bench_fp32c(dev, a, value, work if not a.no_cpu else None, names)
         cubeA = torch.empty((nw, ny, nx), dtype=torch.complex64, device=dev)
         extra["adjoint"] = bench_adjoint(d, vis, cubeA, n2, nx, ny, nw, support, a)
         del cubeA

@@ -160,7 +160,8 @@ std::mutex g_init_mu;
 // hipGetSymbolAddress, and only that pointer reaches kernels.
 constexpr size_t kScratchBytes = LOFAR_STATIC_SCRATCH_BYTES;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
-bool g_profiling = false;
+// stage timing switch (lofar_profile_enable); read by every call's launches
+std::atomic<bool> g_profiling{false};
 
 int fail(int code)
 {
@@ -414,7 +415,7 @@ hipEvent_t pool_event(DeviceCtx& c)
 template <typename F>
 hipError_t stage(DeviceCtx& c, int id, hipStream_t st, F&& fn)
 {
-    if (!g_profiling) return fn();
+    if (!g_profiling.load(std::memory_order_relaxed)) return fn();
     Prof p{id, pool_event(c), pool_event(c)};
     if (p.a) (void)hipEventRecord(p.a, st);
     hipError_t e = fn();
@@ -751,12 +752,10 @@ struct CallGuard {
 
 void report_once(const char* what, int code)
 {
-    static bool printed = false;
-    if (!printed) {
+    static std::atomic<bool> printed{false};
+    if (!printed.exchange(true))
         fprintf(stderr, "liblofar_hip: %s failed: %s (further errors not printed)\n", what,
                 lofar_error_string(code));
-        printed = true;
-    }
 }
 
 // strip width (log2 columns) of the fused path's scratch cube; -1 = row-major
@@ -813,7 +812,7 @@ int test(int nnn) { return nnn + 1; }
 
 int lofar_profile_enable(int on)
 {
-    g_profiling = on != 0;
+    g_profiling.store(on != 0);
     return LOFAR_OK;
 }
 

@@ -121,6 +121,31 @@ def test_grid_is_bitwise_order_independent(cuda, precision, tol):
     assert st == O.OK and rel(a, want) < tol
 
 
+@pytest.mark.parametrize("n,span", [(200000, 50), (20000, 50), (3000, 50), (900, 6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_grid_dense_tile_same_sign(cuda, precision, tol, n, span):
+    """Samples packed inside one 64x64 tile (a span x span square of cells) with
+    visibilities of one sign (positive real, negative imaginary parts), so that cell sums
+    come close to the FP32C packed word's bound and every imaginary part carries into the
+    real half: above 65536 entries, and with more than 256 entries touching a 4x4 block, the
+    tile takes the two-word fixed point; below, the packed word; the cube matches the
+    oracle either way."""
+    from lofar_amd import imager
+    rng = np.random.default_rng(61 + n)
+    nx, nw = 128, 1
+    d = {"uu": rng.uniform(70, 70 + span, n) / nx, "vv": rng.uniform(8, 8 + span, n) / nx,
+         "ww": np.full(n, 0.25), "vis_real": rng.uniform(0.5, 1.5, n).astype(np.float32),
+         "vis_img": -rng.uniform(0.5, 1.5, n).astype(np.float32),
+         "weight": rng.uniform(0.5, 1.0, n).astype(np.float32)}
+    got = H(imager.grid(*dev_inputs(d, cuda), nw, 7, nx, nx, precision=precision))
+    st, want = O.wstack(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / nx, 1 / nw, 7, nx, nx)
+    assert st == O.OK
+    assert rel(got, want) < tol
+    hot = np.abs(want) > 0.5 * np.abs(want).max()
+    assert hot.sum() > 20 and rel(got[hot], want[hot]) < tol
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
 def test_grid_infinite_visibilities(cuda, precision, tol):
     """Infinite visibilities (real parts finite, so not skipped as NaN) propagate into their
@@ -145,11 +170,12 @@ def test_grid_infinite_visibilities(cuda, precision, tol):
     assert rel(got[fin], want[fin]) < tol
 
 
-@pytest.mark.parametrize("grid_cfg", ["1"])
+@pytest.mark.parametrize("grid_cfg", ["1", "2"])
 def test_grid_accumulation_variants(cuda, grid_cfg):
-    """The floating-point accumulation path (LOFAR_GRID_CFG=1, read once per process: LDS
-    ds_add_f64 in both modes instead of exact fixed point) gives the oracle's cube, in a
-    child process with the setting applied."""
+    """The other accumulation paths (LOFAR_GRID_CFG, read once per process: 1 = LDS
+    ds_add_f64 in both modes instead of exact fixed point, 2 = FP32C fixed point with two
+    64-bit words per cell instead of one packed word) give the oracle's cube, in a child
+    process with the setting applied."""
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
