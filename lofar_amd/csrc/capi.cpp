@@ -14,13 +14,36 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
 #include "common.h"
 #include "fft_kernels.h"
 #include "grid_kernels.h"
+
+// A sort of fixed samples kept across calls (lofar_plan_*).  gmem holds the gridding
+// bucket offsets and entries (each entry's sample index in its real part's slot),
+// hmem the degridding bucket offsets, the home entries and the weights; for windows
+// wider than the tile kernels take, hmem holds copies of uu, vv, ww and the weights.
+struct lofar_plan_s {
+    int device = -1, precision = 0, nw = 0, support = 0, nx = 0, ny = 0;
+    long n = 0, F = 1, P = 1;
+    double dx = 0.0, dw = 0.0;
+    bool direct = false;
+    lofar::TileMap gmap{}, hmap{};
+    long long gtotal = 0, htotal = 0;
+    long long* goff = nullptr;
+    void* gent = nullptr;
+    long long* hoff = nullptr;
+    lofar::DEntry* hent = nullptr;
+    float* wt = nullptr;
+    double *uu = nullptr, *vv = nullptr, *ww = nullptr;
+    char* gmem = nullptr;
+    char* hmem = nullptr;
+};
 
 namespace lofar {
 namespace {
@@ -639,16 +662,43 @@ int validate_direct(DeviceCtx& c, Arena* ar, long n, const double* uu, const dou
     return LOFAR_OK;
 }
 
+// plan: the samples' sort from lofar_plan_create (uu, vv, ww unused): its entries get
+// this call's visibilities, without validation, sorting or host synchronisation.
 int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long F, long P,
               const double* uu, const double* vv, const double* ww, const float* vr,
               const float* vi, const float* wt, double dx, double dw, int support, int nx, int ny,
               void* cube, hipStream_t st, unsigned long long* occ = nullptr, int sm_log = -1,
-              Arena* ar = nullptr)
+              Arena* ar = nullptr, const lofar_plan_s* plan = nullptr)
 {
     int r = check_grid_args(precision, nw, n, F, P, dx, dw, support, nx, ny);
     if (r) return fail(r);
-    if (!cube || (n > 0 && (!uu || !vv || !ww || !vr || !vi || !wt))) return fail(LOFAR_EARG);
+    if (!cube || (n > 0 && ((!plan && (!uu || !vv || !ww)) || !vr || !vi || !wt)))
+        return fail(LOFAR_EARG);
     const Geometry g = make_geometry(nw, support, nx, ny, dx, dw);
+    if (plan && plan->direct) {
+        if (occ || sm_log >= 0) return fail(LOFAR_EARG);
+        HIPCHK(stage(c, kStGrid, st, [&] {
+            return grid_direct(precision, n, F, P, plan->uu, plan->vv, plan->ww, vr, vi, wt, g,
+                               cube, accumulate, st);
+        }));
+        return LOFAR_OK;
+    }
+    if (plan) {
+        const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
+        char* eb = nullptr;
+        r = take(c, ar, c.ents, align256((size_t)plan->gtotal * esz) + 256, &eb);
+        if (r) return r;
+        Buckets b{};
+        b.map = plan->gmap;
+        b.offsets = plan->goff;
+        HIPCHK(stage(c, kStScatter, st, [&] {
+            return plan_fill(precision, plan->gtotal, plan->gent, eb, F, P, vr, vi, wt, st);
+        }));
+        HIPCHK(stage(c, kStGrid, st, [&] {
+            return grid_tiles(precision, g, b, eb, cube, accumulate, occ, sm_log, st);
+        }));
+        return LOFAR_OK;
+    }
     if (2 * g.K + 1 > kMaxTileWin) {
         // windows wider than the tile kernels take: validated, then gridded with
         // global atomics, no buckets; the cube is written whole (no occupancy bits)
@@ -907,7 +957,7 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
                       const float* vis_img, const float* weight, double dx, double dw,
                       int w_support, int grid_size_x, int grid_size_y, double resolution,
                       double wmin, double wmax, void* cube, double* image_real,
-                      double* image_imag, int accumulate)
+                      double* image_imag, int accumulate, const lofar_plan_s* plan = nullptr)
 {
     // per-tile occupancy bits: the gridder skips empty tiles and the column pass
     // reads them as zeros (through 32-bit plane offsets, so for planes below 2 GiB;
@@ -937,7 +987,9 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
     // contiguous block and the column transforms read 64-byte runs 1 KiB apart instead
     // of a plane row apart; FP32C keeps row-major planes (no gain measured there).
     // LOFAR_SM_LOG overrides, for measurement (-1: row-major; at most the tile width).
-    const int sm_pref = (direct || generic) ? -1 : sm_log_cfg(precision);
+    // rows of 8192 run the split row passes, which read row-major planes
+    const bool split = grid_size_x >= 8192 && rows_split();
+    const int sm_pref = (direct || generic || split) ? -1 : sm_log_cfg(precision);
     const int sm_log = (sm_pref >= 0 && (1 << sm_pref) <= kTile &&
                         grid_size_x % (1 << sm_pref) == 0 &&
                         (1 << sm_pref) % (precision == LOFAR_FP64 ? 4 : 8) == 0)
@@ -945,7 +997,7 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
                            : -1;
     r = grid_impl(c, precision, 0, num_w_planes, num_points, freq_per_chan, polarizations, uu, vv,
                   ww, vis_real, vis_img, weight, dx, dw, w_support, grid_size_x, grid_size_y, cube,
-                  st, occ, sm_log, ar);
+                  st, occ, sm_log, ar, plan);
     if (r) return r;
     if (generic) {
         r = fft_generic_planes(c, ar, precision, 1, cube, cube, grid_size_x, grid_size_y,
@@ -966,39 +1018,36 @@ static int image_impl(DeviceCtx& c, hipStream_t st, Arena* ar, int precision, in
     const PhaseParams pp =
         phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
     const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
-    if (precision == LOFAR_FP64) {
-        HIPCHK(stage(c, kStCols, st, [&] {
-            return fft_cols<double>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                    1.0, st, occ, sm_log);
+    HIPCHK(stage(c, kStCols, st, [&] {
+        return precision == LOFAR_FP64
+                   ? fft_cols<double>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
+                                      1.0, st, occ, sm_log)
+                   : fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
+                                     1.0, st, occ, sm_log);
+    }));
+    if (split) {
+        // rows of 8192 run the fused pass at 1024 threads a workgroup, where its
+        // per-pixel state does not fit the registers: the row transform (in place)
+        // and the phase/plane sum run as two passes instead
+        const long long lines = (long long)num_w_planes * grid_size_y;
+        HIPCHK(stage(c, kStRows, st, [&] {
+            return precision == LOFAR_FP64
+                       ? fft_rows<double>(1, cube, cube, twx, grid_size_x, lines, scale, st)
+                       : fft_rows<float>(1, cube, cube, twx, grid_size_x, lines, scale, st);
         }));
-        HIPCHK(stage(c, kStRowsPhase, st, [&] {
-            return rows_phase<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
-                                      scale, image_real, image_imag, accumulate, st, sm_log);
+        HIPCHK(stage(c, kStPhase, st, [&] {
+            return phase_sum(precision, cube, image_real, image_imag, grid_size_x, grid_size_y,
+                             num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax,
+                             accumulate, st);
         }));
     } else {
-        HIPCHK(stage(c, kStCols, st, [&] {
-            return fft_cols<float>(1, cube, cube, twy, grid_size_x, grid_size_y, num_w_planes,
-                                   1.0, st, occ, sm_log);
+        HIPCHK(stage(c, kStRowsPhase, st, [&] {
+            return precision == LOFAR_FP64
+                       ? rows_phase<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
+                                            scale, image_real, image_imag, accumulate, st, sm_log)
+                       : rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
+                                           scale, image_real, image_imag, accumulate, st, sm_log);
         }));
-        if (grid_size_x >= 8192 && sm_log < 0 && rows_split()) {
-            // rows of 8192 run the fused pass at 1024 threads a workgroup, where its
-            // per-pixel state does not fit the registers: the row transform (in
-            // place) and the phase/plane sum run as two passes instead
-            HIPCHK(stage(c, kStRows, st, [&] {
-                return fft_rows<float>(1, cube, cube, twx, grid_size_x,
-                                       (long long)num_w_planes * grid_size_y, scale, st);
-            }));
-            HIPCHK(stage(c, kStPhase, st, [&] {
-                return phase_sum(LOFAR_FP32C, cube, image_real, image_imag, grid_size_x,
-                                 grid_size_y, num_w_planes, grid_size_x, grid_size_y, resolution,
-                                 wmin, wmax, accumulate, st);
-            }));
-        } else {
-            HIPCHK(stage(c, kStRowsPhase, st, [&] {
-                return rows_phase<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes, pp,
-                                         scale, image_real, image_imag, accumulate, st, sm_log);
-            }));
-        }
     }
     return LOFAR_OK;
 }
@@ -1094,6 +1143,77 @@ int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long
     return LOFAR_OK;
 }
 
+// The transforms of the adjoint (lofar_degrid_dev) up to the interpolation: conjugate
+// phase screen, forward row and column transforms with the 1/(nx*ny), into `cube`.
+static int degrid_transform(DeviceCtx& c, hipStream_t st, int precision, int num_w_planes,
+                            int grid_size_x, int grid_size_y, double resolution, double wmin,
+                            double wmax, const double* image_real, const double* image_imag,
+                            void* cube)
+{
+    int r = LOFAR_OK;
+    void *twx, *twy;
+    r = twiddles(c, grid_size_x, precision, &twx);
+    if (r) return r;
+    r = twiddles(c, grid_size_y, precision, &twy);
+    if (r) return r;
+    const PhaseParams pp =
+        phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
+    const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
+    if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) {
+        // sides that are not powers of two: conjugate screen, then the mixed-radix
+        // forward transforms (rows, then columns with the 1/(nx*ny))
+        HIPCHK(stage(c, kStRowsPhaseAdj, st, [&] {
+            return phase_adjoint(precision, image_real, image_imag, cube, grid_size_x,
+                                 grid_size_y, num_w_planes, grid_size_x, grid_size_y,
+                                 resolution, wmin, wmax, st);
+        }));
+        r = fft_generic_planes(c, nullptr, precision, -1, cube, cube, grid_size_x,
+                               grid_size_y, num_w_planes, scale, st, true);
+        if (r) return r;
+    } else if (grid_size_x >= 8192 && rows_split()) {
+        // rows of 8192: the fused adjoint row pass does not fit its 1024-thread
+        // workgroups' registers, so the conjugate screen (every plane written) and
+        // the forward row transform (in place) run as two passes
+        HIPCHK(stage(c, kStRowsPhaseAdj, st, [&] {
+            const hipError_t e =
+                phase_adjoint(precision, image_real, image_imag, cube, grid_size_x,
+                              grid_size_y, num_w_planes, grid_size_x, grid_size_y,
+                              resolution, wmin, wmax, st);
+            if (e != hipSuccess) return e;
+            const long long lines = (long long)num_w_planes * grid_size_y;
+            return precision == LOFAR_FP64
+                       ? fft_rows<double>(-1, cube, cube, twx, grid_size_x, lines, 1.0, st)
+                       : fft_rows<float>(-1, cube, cube, twx, grid_size_x, lines, 1.0, st);
+        }));
+        HIPCHK(stage(c, kStCols, st, [&] {
+            return precision == LOFAR_FP64
+                       ? fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                          num_w_planes, scale, st)
+                       : fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                         num_w_planes, scale, st);
+        }));
+    } else if (precision == LOFAR_FP64) {
+        HIPCHK(stage(c, kStRowsPhaseAdj, st, [&] {
+            return rows_phase_adj<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
+                                          pp, 1.0, image_real, image_imag, st);
+        }));
+        HIPCHK(stage(c, kStCols, st, [&] {
+            return fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                    num_w_planes, scale, st);
+        }));
+    } else {
+        HIPCHK(stage(c, kStRowsPhaseAdj, st, [&] {
+            return rows_phase_adj<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
+                                         pp, 1.0, image_real, image_imag, st);
+        }));
+        HIPCHK(stage(c, kStCols, st, [&] {
+            return fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
+                                   num_w_planes, scale, st);
+        }));
+    }
+    return LOFAR_OK;
+}
+
 int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq_per_chan,
                      long polarizations, const double* uu, const double* vv, const double* ww,
                      const float* weight, double dx, double dw, int w_support, int grid_size_x,
@@ -1109,71 +1229,209 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
     {
         CallGuard cg(stream);
         if (cg.status) return cg.status;
-        DeviceCtx& c = *cg.c;
-        void *twx, *twy;
-        r = twiddles(c, grid_size_x, precision, &twx);
+        r = degrid_transform(*cg.c, cg.st, precision, num_w_planes, grid_size_x, grid_size_y,
+                             resolution, wmin, wmax, image_real, image_imag, cube);
         if (r) return r;
-        r = twiddles(c, grid_size_y, precision, &twy);
-        if (r) return r;
-        const PhaseParams pp =
-            phase_params(num_w_planes, grid_size_x, grid_size_y, resolution, wmin, wmax);
-        const double scale = 1.0 / ((double)grid_size_x * (double)grid_size_y);
-        if (!pow2_ok(grid_size_x) || !pow2_ok(grid_size_y)) {
-            // sides that are not powers of two: conjugate screen, then the mixed-radix
-            // forward transforms (rows, then columns with the 1/(nx*ny))
-            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
-                return phase_adjoint(precision, image_real, image_imag, cube, grid_size_x,
-                                     grid_size_y, num_w_planes, grid_size_x, grid_size_y,
-                                     resolution, wmin, wmax, cg.st);
-            }));
-            r = fft_generic_planes(c, nullptr, precision, -1, cube, cube, grid_size_x,
-                                   grid_size_y, num_w_planes, scale, cg.st, true);
-            if (r) return r;
-        } else if (grid_size_x >= 8192 && rows_split()) {
-            // rows of 8192: the fused adjoint row pass does not fit its 1024-thread
-            // workgroups' registers, so the conjugate screen (every plane written) and
-            // the forward row transform (in place) run as two passes
-            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
-                const hipError_t e =
-                    phase_adjoint(precision, image_real, image_imag, cube, grid_size_x,
-                                  grid_size_y, num_w_planes, grid_size_x, grid_size_y,
-                                  resolution, wmin, wmax, cg.st);
-                if (e != hipSuccess) return e;
-                const long long lines = (long long)num_w_planes * grid_size_y;
-                return precision == LOFAR_FP64
-                           ? fft_rows<double>(-1, cube, cube, twx, grid_size_x, lines, 1.0, cg.st)
-                           : fft_rows<float>(-1, cube, cube, twx, grid_size_x, lines, 1.0, cg.st);
-            }));
-            HIPCHK(stage(c, kStCols, cg.st, [&] {
-                return precision == LOFAR_FP64
-                           ? fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                              num_w_planes, scale, cg.st)
-                           : fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                             num_w_planes, scale, cg.st);
-            }));
-        } else if (precision == LOFAR_FP64) {
-            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
-                return rows_phase_adj<double>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
-                                              pp, 1.0, image_real, image_imag, cg.st);
-            }));
-            HIPCHK(stage(c, kStCols, cg.st, [&] {
-                return fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                        num_w_planes, scale, cg.st);
-            }));
-        } else {
-            HIPCHK(stage(c, kStRowsPhaseAdj, cg.st, [&] {
-                return rows_phase_adj<float>(cube, twx, grid_size_x, grid_size_y, num_w_planes,
-                                             pp, 1.0, image_real, image_imag, cg.st);
-            }));
-            HIPCHK(stage(c, kStCols, cg.st, [&] {
-                return fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                       num_w_planes, scale, cg.st);
-            }));
-        }
     }
     return lofar_degrid_cube_dev(precision, num_w_planes, num_points, freq_per_chan, polarizations,
                                  uu, vv, ww, weight, dx, dw, w_support, grid_size_x, grid_size_y,
                                  cube, model_real, model_imag, stream);
+}
+
+// ---------------------------------------------------------------------------
+// plans: one sort of fixed samples, kept for repeated imaging and degridding
+
+int lofar_plan_create(int precision, int num_w_planes, long num_points, long freq_per_chan,
+                      long polarizations, const double* uu, const double* vv, const double* ww,
+                      const float* weight, double dx, double dw, int w_support, int grid_size_x,
+                      int grid_size_y, void* stream, lofar_plan* plan)
+{
+    if (!plan) return fail(LOFAR_EARG);
+    *plan = nullptr;
+    if (!fft_size_ok(grid_size_x) || !fft_size_ok(grid_size_y)) return fail(LOFAR_ESIZE);
+    int r = check_grid_args(precision, num_w_planes, num_points, freq_per_chan, polarizations, dx,
+                            dw, w_support, grid_size_x, grid_size_y);
+    if (r) return fail(r);
+    if (num_points > 0 && (!uu || !vv || !ww || !weight)) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    const hipStream_t st = cg.st;
+    std::unique_ptr<lofar_plan_s, int (*)(lofar_plan)> p(new (std::nothrow) lofar_plan_s(),
+                                                         [](lofar_plan q) {
+                                                             if (q) {
+                                                                 (void)hipFree(q->gmem);
+                                                                 (void)hipFree(q->hmem);
+                                                             }
+                                                             delete q;
+                                                             return 0;
+                                                         });
+    if (!p) return fail(LOFAR_ENOMEM);
+    p->device = c.device;
+    p->precision = precision;
+    p->nw = num_w_planes;
+    p->support = w_support;
+    p->nx = grid_size_x;
+    p->ny = grid_size_y;
+    p->n = num_points;
+    p->F = freq_per_chan;
+    p->P = polarizations;
+    p->dx = dx;
+    p->dw = dw;
+    const Geometry g = make_geometry(num_w_planes, w_support, grid_size_x, grid_size_y, dx, dw);
+    p->direct = 2 * g.K + 1 > kMaxTileWin;
+    const size_t wbytes = align256((size_t)num_points * polarizations * sizeof(float));
+    auto alloc = [&](char** mem, size_t bytes) {
+        if (hipMalloc((void**)mem, bytes + 256) != hipSuccess) {
+            *mem = nullptr;
+            (void)hipGetLastError();
+            return fail(LOFAR_ENOMEM);
+        }
+        return (int)LOFAR_OK;
+    };
+    if (p->direct) {
+        r = validate_direct(c, nullptr, num_points, uu, vv, ww, g, st);
+        if (r) return r;
+        const size_t cb = align256((size_t)num_points * sizeof(double));
+        r = alloc(&p->hmem, 3 * cb + wbytes);
+        if (r) return r;
+        p->uu = (double*)p->hmem;
+        p->vv = (double*)(p->hmem + cb);
+        p->ww = (double*)(p->hmem + 2 * cb);
+        p->wt = (float*)(p->hmem + 3 * cb);
+        if (num_points > 0) {
+            HIPCHK(hipMemcpyAsync(p->uu, uu, num_points * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(p->vv, vv, num_points * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(p->ww, ww, num_points * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(p->wt, weight, num_points * polarizations * sizeof(float),
+                                  hipMemcpyDeviceToDevice, st));
+        }
+    } else {
+        // gridding: the sort of lofar_grid_dev with sample indices for visibilities
+        Buckets b;
+        long long total = 0;
+        r = build_buckets(c, kBucketFootprint, num_points, uu, vv, ww, g, st, b, total);
+        if (r) return r;
+        const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
+        const size_t rbytes = align256((size_t)total * sizeof(Entry64));
+        const size_t ebytes = align256((size_t)total * esz);
+        const size_t l2 = level2_layout(b, total, nullptr);
+        r = ensure(c, c.ents, rbytes + ebytes + l2 + 256);
+        if (r) return r;
+        char* eb = (char*)c.ents.ptr;
+        level2_layout(b, total, eb + rbytes + ebytes);
+        HIPCHK(bucket_sort_grid(precision, num_points, freq_per_chan, polarizations, uu, vv, ww,
+                                nullptr, nullptr, nullptr, g, b, (Entry64*)eb, eb + rbytes, st,
+                                true));
+        const long long nbg = b.map.nbuckets();
+        const size_t obytes = align256((size_t)(nbg + 1) * sizeof(long long));
+        r = alloc(&p->gmem, obytes + ebytes);
+        if (r) return r;
+        p->gmap = b.map;
+        p->gtotal = total;
+        p->goff = (long long*)p->gmem;
+        p->gent = p->gmem + obytes;
+        HIPCHK(hipMemcpyAsync(p->goff, b.offsets, (nbg + 1) * sizeof(long long),
+                              hipMemcpyDeviceToDevice, st));
+        if (total > 0)
+            HIPCHK(hipMemcpyAsync(p->gent, eb + rbytes, total * esz, hipMemcpyDeviceToDevice, st));
+        // the home sort below reuses (and may reallocate) the workspace copied from
+        HIPCHK(hipStreamSynchronize(st));
+        // degridding: the home sort of lofar_degrid_cube_dev
+        Buckets hb;
+        long long htotal = 0;
+        r = build_buckets(c, kBucketHome, num_points, uu, vv, ww, g, st, hb, htotal);
+        if (r) return r;
+        const size_t hbytes = align256((size_t)htotal * sizeof(DEntry));
+        const size_t hl2 = level2_layout(hb, htotal, nullptr);
+        r = ensure(c, c.ents, 2 * hbytes + hl2 + 256);
+        if (r) return r;
+        char* hbuf = (char*)c.ents.ptr;
+        level2_layout(hb, htotal, hbuf + 2 * hbytes);
+        HIPCHK(bucket_sort_home(num_points, uu, vv, ww, polarizations == 1 ? weight : nullptr, g,
+                                hb, (DEntry*)hbuf, (DEntry*)(hbuf + hbytes), st));
+        const long long nbh = hb.map.nbuckets();
+        const size_t hobytes = align256((size_t)(nbh + 1) * sizeof(long long));
+        r = alloc(&p->hmem, hobytes + hbytes + wbytes);
+        if (r) return r;
+        p->hmap = hb.map;
+        p->htotal = htotal;
+        p->hoff = (long long*)p->hmem;
+        p->hent = (DEntry*)(p->hmem + hobytes);
+        p->wt = (float*)(p->hmem + hobytes + hbytes);
+        HIPCHK(hipMemcpyAsync(p->hoff, hb.offsets, (nbh + 1) * sizeof(long long),
+                              hipMemcpyDeviceToDevice, st));
+        if (htotal > 0)
+            HIPCHK(hipMemcpyAsync(p->hent, hbuf + hbytes, htotal * sizeof(DEntry),
+                                  hipMemcpyDeviceToDevice, st));
+        if (num_points > 0)
+            HIPCHK(hipMemcpyAsync(p->wt, weight, num_points * polarizations * sizeof(float),
+                                  hipMemcpyDeviceToDevice, st));
+    }
+    // the caller may change or free its arrays once this returns
+    HIPCHK(hipStreamSynchronize(st));
+    *plan = p.release();
+    return LOFAR_OK;
+}
+
+int lofar_plan_image(lofar_plan plan, const float* vis_real, const float* vis_img,
+                     const float* weight, double resolution, double wmin, double wmax,
+                     void* cube, double* image_real, double* image_imag, int accumulate,
+                     void* stream)
+{
+    if (!plan || !image_real || !image_imag) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    if (cg.c->device != plan->device) return fail(LOFAR_EARG);
+    return image_impl(*cg.c, cg.st, nullptr, plan->precision, plan->nw, plan->n, plan->F,
+                      plan->P, nullptr, nullptr, nullptr, vis_real, vis_img, weight, plan->dx,
+                      plan->dw, plan->support, plan->nx, plan->ny, resolution, wmin, wmax, cube,
+                      image_real, image_imag, accumulate, plan);
+}
+
+int lofar_plan_degrid(lofar_plan plan, double resolution, double wmin, double wmax,
+                      const double* image_real, const double* image_imag, void* cube,
+                      void* model_real, void* model_imag, void* stream)
+{
+    if (!plan || !cube || !image_real || !image_imag) return fail(LOFAR_EARG);
+    if (plan->n > 0 && (!model_real || !model_imag)) return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    if (cg.c->device != plan->device) return fail(LOFAR_EARG);
+    DeviceCtx& c = *cg.c;
+    int r = degrid_transform(c, cg.st, plan->precision, plan->nw, plan->nx, plan->ny, resolution,
+                             wmin, wmax, image_real, image_imag, cube);
+    if (r) return r;
+    const Geometry g = make_geometry(plan->nw, plan->support, plan->nx, plan->ny, plan->dx,
+                                     plan->dw);
+    if (plan->direct) {
+        HIPCHK(stage(c, kStDegrid, cg.st, [&] {
+            return degrid_direct(plan->precision, plan->n, plan->F, plan->P, plan->uu, plan->vv,
+                                 plan->ww, plan->wt, g, cube, model_real, model_imag, cg.st);
+        }));
+        return LOFAR_OK;
+    }
+    Buckets hb{};
+    hb.map = plan->hmap;
+    hb.offsets = plan->hoff;
+    HIPCHK(stage(c, kStDegrid, cg.st, [&] {
+        return degrid_tiles(plan->precision, g, hb, plan->hent, cube, plan->wt, plan->F, plan->P,
+                            model_real, model_imag, cg.st);
+    }));
+    return LOFAR_OK;
+}
+
+int lofar_plan_destroy(lofar_plan plan)
+{
+    if (!plan) return LOFAR_OK;
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != plan->device) (void)hipSetDevice(plan->device);
+    // hipFree waits for the device's queued work, which may still read the plan
+    (void)hipFree(plan->gmem);
+    (void)hipFree(plan->hmem);
+    if (cur >= 0 && cur != plan->device) (void)hipSetDevice(cur);
+    delete plan;
+    return LOFAR_OK;
 }
 
 // ---------------------------------------------------------------------------

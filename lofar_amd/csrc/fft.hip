@@ -493,6 +493,12 @@ static hipError_t launch_cols_n(int sign, const void* in, void* out, const void*
     constexpr int B = Plan<N, 16>::LDS * 2 * (int)sizeof(Real);
     constexpr int W2 = 140000 / B >= 1 ? (140000 / B > 16 ? 16 : 140000 / B) : 1;
     constexpr int W64 = 64 / (2 * (int)sizeof(Real));
+    // 8192-long complex-double columns at 32 elements a thread would need 128 VGPRs
+    // of data alone at 1024 threads (they spilled: 0.74 TB/s); 16 elements, 2 columns
+    // and 2 LDS rounds run without spills (2.5 TB/s)
+    if constexpr (sizeof(Real) == 8 && N >= 8192)
+        if (col_cfg() == 0 || sm_log >= 0)
+            return launch_cols_cfg<Real, N, 16, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
     if (sm_log >= 0)
         return launch_cols_cfg<Real, N, 32, W64, 2>(sign, in, out, tw, nx, nw, scale, occ, sm_log, st);
     switch (col_cfg()) {

@@ -472,9 +472,9 @@ def test_image_ignores_stale_scratch_cube(cuda, precision, tol):
 
 
 def test_image_8192_split_row_pass(cuda):
-    """8192-wide FP32C grids run the row transform and the phase/plane sum as two passes
-    (the fused row pass does not fit its 1024-thread workgroups): the image agrees with the
-    FP64 mode (fused path) on the same samples, and accumulate mode adds to the image."""
+    """8192-wide grids run the row transform and the phase/plane sum as two passes in both
+    modes (the fused row pass does not fit its 1024-thread workgroups): the FP32C image
+    agrees with the FP64 one on the same samples, and accumulate mode adds to the image."""
     from lofar_amd import imager
     nx, nw = 8192, 3
     d = synth(200000, nx, nw, seed=8192)

@@ -92,9 +92,9 @@ def test_config3_fp32c_64M_4096x32_matches_reference(cuda):
 
 
 def test_8192x2_fp64_matches_reference(cuda):
-    """8192-wide planes in FP64: the fused row pass at 8192 and the column transform's
-    second occupancy word (tile rows 64..127 exist only at 8192) against the reference
-    path; 2M samples on 2 w-planes, support 7."""
+    """8192-wide planes in FP64: the 16-element complex-double column transform with the
+    second occupancy word (tile rows 64..127 exist only at 8192), the split row transform
+    and phase sum, against the reference path; 2M samples on 2 w-planes, support 7."""
     n, nx, nw, support = 2 * 1024 * 1024, 8192, 2, 7
     d = _obs(n, nx, nw, 8192, cuda)
     got64 = _image(d, nw, support, nx, "fp64")
