@@ -163,6 +163,38 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
                      const double* image_real, const double* image_imag, void* cube,
                      void* model_real, void* model_imag, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* 4. plans: one sort of fixed samples, kept for repeated calls        */
+
+/* No counterpart in the reference, which grids once per run
+ * (gridding.c:245-261): iterative callers (chi^2 fits, major cycles) grid and
+ * degrid the same (u, v, w) many times with new visibilities or a new image.
+ * lofar_plan_create validates and sorts the samples once, for gridding (each
+ * sample's index in (w, tile) order) and for degridding (home tiles), into device
+ * memory the plan owns (about 20 bytes per sample in FP32C mode, 32 in FP64, plus
+ * 32 per sample for degridding), and copies the weights it needs; the caller's
+ * arrays may change or be freed afterwards.  lofar_plan_image and
+ * lofar_plan_degrid then run lofar_image_dev and lofar_degrid_dev on the planned
+ * samples without sorting and without host synchronisation: the same results, bit
+ * for bit.  The degridding weights are those given to lofar_plan_create; the
+ * imaging weights come with each call.  Windows wider than 17 cells keep copies of
+ * uu, vv, ww and take the unsorted paths.  A plan belongs to the device current at
+ * its creation (LOFAR_EARG elsewhere). */
+typedef struct lofar_plan_s* lofar_plan;
+int lofar_plan_create(int precision, int num_w_planes, long num_points, long freq_per_chan,
+                      long polarizations, const double* uu, const double* vv, const double* ww,
+                      const float* weight, double dx, double dw, int w_support, int grid_size_x,
+                      int grid_size_y, void* stream, lofar_plan* plan);
+int lofar_plan_image(lofar_plan plan, const float* vis_real, const float* vis_img,
+                     const float* weight, double resolution, double wmin, double wmax,
+                     void* cube, double* image_real, double* image_imag, int accumulate,
+                     void* stream);
+int lofar_plan_degrid(lofar_plan plan, double resolution, double wmin, double wmax,
+                      const double* image_real, const double* image_imag, void* cube,
+                      void* model_real, void* model_imag, void* stream);
+/* Frees the plan's device memory after the work queued on it has finished. */
+int lofar_plan_destroy(lofar_plan plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,11 @@ SIGNATURES = {
                                    _vp, _vp, _vp]),
     "lofar_degrid_dev": (_i, [_i, _i, _l, _l, _l, _vp, _vp, _vp, _vp, _d, _d, _i, _i, _i, _d, _d,
                               _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lofar_plan_create": (_i, [_i, _i, _l, _l, _l, _vp, _vp, _vp, _vp, _d, _d, _i, _i, _i, _vp,
+                               C.POINTER(_vp)]),
+    "lofar_plan_image": (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _i, _vp]),
+    "lofar_plan_degrid": (_i, [_vp, _d, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lofar_plan_destroy": (_i, [_vp]),
 }
 
 _lib = None

@@ -43,10 +43,14 @@ class ImagingOperator:
 
     uu, vv, ww: float64 CUDA tensors [N] normalised as the reference's inputs;
     weight: float32 CUDA tensor [N] or None (unit weights); cube: optional scratch
-    [nw, ny, nx] of the precision's complex dtype."""
+    [nw, ny, nx] of the precision's complex dtype.  With ``plan`` (the default) the
+    samples are sorted once, at the first application (``imager.Plan``), and every
+    later forward and adjoint reuses that sort: the same results without re-sorting
+    or host synchronisation; the operator then assumes uu, vv, ww and weight do not
+    change."""
 
     def __init__(self, uu, vv, ww, weight, num_w_planes, w_support, grid_size_x, grid_size_y,
-                 resolution, wmin, wmax, precision="fp64", cube=None):
+                 resolution, wmin, wmax, precision="fp64", cube=None, plan=True):
         torch = _torch()
         self.uu, self.vv, self.ww = uu, vv, ww
         n = uu.numel()
@@ -61,16 +65,38 @@ class ImagingOperator:
             cube = torch.empty((num_w_planes, grid_size_y, grid_size_x),
                                dtype=imager.cube_dtype(self.precision), device=uu.device)
         self.cube = cube  # scratch shared by forward and adjoint
+        self._use_plan = plan
+        self._plan = None
 
     @property
     def num_samples(self):
         return self.uu.numel()
 
+    def plan(self):
+        """The samples' sort (created at the first call), or None without plans."""
+        if self._use_plan and self._plan is None:
+            self._plan = imager.Plan(self.uu, self.vv, self.ww, self.weight, self.nw,
+                                     self.support, self.nx, self.ny, precision=self.precision)
+        return self._plan
+
     def forward(self, img_re, img_im):
         """Model visibilities (float64 in FP64 mode, float32 in FP32C mode)."""
+        pl = self.plan()
+        if pl is not None:
+            return pl.degrid(img_re.contiguous(), img_im.contiguous(), self.resolution,
+                             self.wmin, self.wmax, cube=self.cube)
         return imager.degrid(img_re.contiguous(), img_im.contiguous(), self.uu, self.vv, self.ww,
                              self.weight, self.nw, self.support, self.resolution, self.wmin,
                              self.wmax, precision=self.precision, cube=self.cube)
+
+    def _image(self, vr, vi, **kw):
+        pl = self.plan()
+        if pl is not None:
+            return pl.image(vr, vi, self.unit, self.resolution, self.wmin, self.wmax,
+                            cube=self.cube, **kw)
+        return imager.image(self.uu, self.vv, self.ww, vr, vi, self.unit, self.nw, self.support,
+                            self.nx, self.ny, self.resolution, self.wmin, self.wmax,
+                            precision=self.precision, cube=self.cube, **kw)
 
     def adjoint(self, vis_re, vis_im):
         """Dirty image (float64 re, im) of the weighted visibilities: A^H v."""
@@ -79,15 +105,11 @@ class ImagingOperator:
         gr = vis_re.to(torch.float64) * w
         gi = vis_im.to(torch.float64) * w
         hr, hi = gr.to(torch.float32), gi.to(torch.float32)
-        args = (self.nw, self.support, self.nx, self.ny, self.resolution, self.wmin, self.wmax)
-        re, im = imager.image(self.uu, self.vv, self.ww, hr.contiguous(), hi.contiguous(),
-                              self.unit, *args, precision=self.precision, cube=self.cube)
+        re, im = self._image(hr.contiguous(), hi.contiguous())
         if self.precision == FP64:
             lr = (gr - hr.to(torch.float64)).to(torch.float32).contiguous()
             li = (gi - hi.to(torch.float64)).to(torch.float32).contiguous()
-            imager.image(self.uu, self.vv, self.ww, lr, li, self.unit, *args,
-                         precision=self.precision, cube=self.cube, image_real=re,
-                         image_imag=im, accumulate=True)
+            self._image(lr, li, image_real=re, image_imag=im, accumulate=True)
         return re, im
 
 

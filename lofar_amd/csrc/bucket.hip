@@ -182,7 +182,9 @@ __global__ __launch_bounds__(256) void k_scan_add(long long* out, const long lon
 // Rec: Entry64 (FP64 mode) or Rec32 (FP32 mode, position relative to the
 // super-tile, rec_axis)
 // GLOBAL: base1 is one global cursor per super-tile (zeroed), taken with global atomics.
-template <typename Rec, bool GLOBAL>
+// IDX: the records carry the sample's index (bits of the real part's slot, entry_index)
+// instead of its visibility, for a sort kept across calls (plan_fill).
+template <typename Rec, bool GLOBAL, bool IDX>
 __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long F, long long P,
                                                          const double* uu, const double* vv,
                                                          const double* ww, const float* vr,
@@ -211,7 +213,12 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
             const long long i = min(i0 + (long long)u * 256, end - 1);
             ok[u] = i0 + (long long)u * 256 < end && sample_window(uu[i], vv[i], ww[i], g, s[u]) &&
                     s[u].j0 <= s[u].j1 && s[u].k0 <= s[u].k1;
-            sample_vis(i, F, P, vr, vi, wt, e[u].vr, e[u].vi);
+            if constexpr (IDX) {
+                e[u].vr = 0.0;
+                e[u].vi = 0.0;
+            } else {
+                sample_vis(i, F, P, vr, vi, wt, e[u].vr, e[u].vi);
+            }
             e[u].pu = s[u].pu;
             e[u].pv = s[u].pv;
         }
@@ -227,11 +234,13 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
                         Rec32 r;
                         r.u = rec_axis(e[u].pu, (su << tm.lsst) * kTile, tm.rfb);
                         r.v = rec_axis(e[u].pv, (sv << tm.lsst) * kTile, tm.rfb);
-                        r.vr = (float)e[u].vr;
+                        r.vr = IDX ? __uint_as_float((unsigned)(i0 + u * 256)) : (float)e[u].vr;
                         r.vi = (float)e[u].vi;
                         rec[at] = r;
                     } else {
-                        rec[at] = e[u];
+                        Entry64 r = e[u];
+                        if constexpr (IDX) r.vr = __longlong_as_double(i0 + u * 256);
+                        rec[at] = r;
                     }
                 }
         }
@@ -456,8 +465,8 @@ __global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, cons
                     Entry32 e;
                     e.wu = pack_cell(s.gu - u0, pu - (double)s.gu);
                     e.wv = pack_cell(s.gv - v0, pv - (double)s.gv);
-                    e.vr = (float)x.vr;
-                    e.vi = (float)x.vi;
+                    e.vr = x.vr;  // bit for bit (a plan's index, see k_l1_scatter_grid)
+                    e.vi = x.vi;
                     ent[p] = e;
                 } else {
                     ent[p] = x;
@@ -535,10 +544,25 @@ static hipError_t level2(const Geometry& g, const Buckets& b, const Rec* rec, Ou
     return hipGetLastError();
 }
 
+template <typename Rec>
+static void launch_l1_scatter_grid(bool global_l1, bool index_vis, size_t lds, hipStream_t st,
+                                   long long n, long long F, long long P, const double* uu,
+                                   const double* vv, const double* ww, const float* vr,
+                                   const float* vi, const float* wt, const Geometry& g,
+                                   const Buckets& b, Rec* rec)
+{
+    auto k = global_l1 ? (index_vis ? k_l1_scatter_grid<Rec, true, true>
+                                    : k_l1_scatter_grid<Rec, true, false>)
+                       : (index_vis ? k_l1_scatter_grid<Rec, false, true>
+                                    : k_l1_scatter_grid<Rec, false, false>);
+    hipLaunchKernelGGL(k, dim3((unsigned)b.nwg), dim3(256), lds, st, n, F, P, uu, vv, ww, vr, vi,
+                       wt, g, b.map, b.chunk, b.off1, b.base1, rec);
+}
+
 hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P,
                             const double* uu, const double* vv, const double* ww, const float* vr,
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
-                            Entry64* rec, void* entries, hipStream_t st)
+                            Entry64* rec, void* entries, hipStream_t st, bool index_vis)
 {
     const size_t lds = b.global_l1 ? 0 : (size_t)b.map.nk1 * sizeof(unsigned);
     if (n > 0 && b.global_l1) {
@@ -546,20 +570,55 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
         if (e != hipSuccess) return e;
     }
     if (precision == LOFAR_FP64) {
-        if (n > 0) {
-            auto k = b.global_l1 ? k_l1_scatter_grid<Entry64, true> : k_l1_scatter_grid<Entry64, false>;
-            hipLaunchKernelGGL(k, dim3((unsigned)b.nwg), dim3(256), lds, st, n, F, P, uu, vv, ww,
-                               vr, vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec);
-        }
+        if (n > 0)
+            launch_l1_scatter_grid<Entry64>(b.global_l1, index_vis, lds, st, n, F, P, uu, vv, ww,
+                                            vr, vi, wt, g, b, rec);
         return level2<kOutFp64>(g, b, (const Entry64*)rec, (Entry64*)entries, st);
     }
     Rec32* rec32 = (Rec32*)rec;  // the record buffer holds total_rec 32-byte records
-    if (n > 0) {
-        auto k = b.global_l1 ? k_l1_scatter_grid<Rec32, true> : k_l1_scatter_grid<Rec32, false>;
-        hipLaunchKernelGGL(k, dim3((unsigned)b.nwg), dim3(256), lds, st, n, F, P, uu, vv, ww, vr,
-                           vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec32);
-    }
+    if (n > 0)
+        launch_l1_scatter_grid<Rec32>(b.global_l1, index_vis, lds, st, n, F, P, uu, vv, ww, vr,
+                                      vi, wt, g, b, rec32);
     return level2<kOutFp32>(g, b, (const Rec32*)rec32, (Entry32*)entries, st);
+}
+
+// A plan's entries, with this call's visibilities: each entry's sample index (left in
+// its real part's slot by the sort) replaced by the sample's channel/polarisation-
+// summed weighted visibility, as k_l1_scatter_grid forms it.
+template <typename Ent>
+__global__ __launch_bounds__(256) void k_plan_fill(long long total, const Ent* pent, Ent* ent,
+                                                   long long F, long long P, const float* vr,
+                                                   const float* vi, const float* wt)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    Ent x = pent[e];
+    double sr, si;
+    if constexpr (sizeof(Ent) == sizeof(Entry64)) {
+        sample_vis(__double_as_longlong(x.vr), F, P, vr, vi, wt, sr, si);
+        x.vr = sr;
+        x.vi = si;
+    } else {
+        sample_vis((long long)__float_as_uint(x.vr), F, P, vr, vi, wt, sr, si);
+        x.vr = (float)sr;
+        x.vi = (float)si;
+    }
+    ent[e] = x;
+}
+
+hipError_t plan_fill(int precision, long long total, const void* pent, void* ent, long long F,
+                     long long P, const float* vr, const float* vi, const float* wt,
+                     hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    const dim3 grid(blocks_for(total, 256));
+    if (precision == LOFAR_FP64)
+        hipLaunchKernelGGL((k_plan_fill<Entry64>), grid, dim3(256), 0, st, total,
+                           (const Entry64*)pent, (Entry64*)ent, F, P, vr, vi, wt);
+    else
+        hipLaunchKernelGGL((k_plan_fill<Entry32>), grid, dim3(256), 0, st, total,
+                           (const Entry32*)pent, (Entry32*)ent, F, P, vr, vi, wt);
+    return hipGetLastError();
 }
 
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,

@@ -39,6 +39,7 @@ struct lofar_plan_s {
     void* gent = nullptr;
     long long* hoff = nullptr;
     lofar::DEntry* hent = nullptr;
+    unsigned long long* need = nullptr;  // tiles the degridding reads (powers of two)
     float* wt = nullptr;
     double *uu = nullptr, *vv = nullptr, *ww = nullptr;
     char* gmem = nullptr;
@@ -1145,10 +1146,12 @@ int lofar_degrid_cube_dev(int precision, int num_w_planes, long num_points, long
 
 // The transforms of the adjoint (lofar_degrid_dev) up to the interpolation: conjugate
 // phase screen, forward row and column transforms with the 1/(nx*ny), into `cube`.
+// need (optional): the tiles the interpolation reads (need_tiles); the last column
+// transform stores only those.
 static int degrid_transform(DeviceCtx& c, hipStream_t st, int precision, int num_w_planes,
                             int grid_size_x, int grid_size_y, double resolution, double wmin,
                             double wmax, const double* image_real, const double* image_imag,
-                            void* cube)
+                            void* cube, const unsigned long long* need = nullptr)
 {
     int r = LOFAR_OK;
     void *twx, *twy;
@@ -1188,9 +1191,9 @@ static int degrid_transform(DeviceCtx& c, hipStream_t st, int precision, int num
         HIPCHK(stage(c, kStCols, st, [&] {
             return precision == LOFAR_FP64
                        ? fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                          num_w_planes, scale, st)
+                                          num_w_planes, scale, st, need)
                        : fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                         num_w_planes, scale, st);
+                                         num_w_planes, scale, st, need);
         }));
     } else if (precision == LOFAR_FP64) {
         HIPCHK(stage(c, kStRowsPhaseAdj, st, [&] {
@@ -1199,7 +1202,7 @@ static int degrid_transform(DeviceCtx& c, hipStream_t st, int precision, int num
         }));
         HIPCHK(stage(c, kStCols, st, [&] {
             return fft_cols<double>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                    num_w_planes, scale, st);
+                                    num_w_planes, scale, st, need);
         }));
     } else {
         HIPCHK(stage(c, kStRowsPhaseAdj, st, [&] {
@@ -1208,7 +1211,7 @@ static int degrid_transform(DeviceCtx& c, hipStream_t st, int precision, int num
         }));
         HIPCHK(stage(c, kStCols, st, [&] {
             return fft_cols<float>(-1, cube, cube, twy, grid_size_x, grid_size_y,
-                                   num_w_planes, scale, st);
+                                   num_w_planes, scale, st, need);
         }));
     }
     return LOFAR_OK;
@@ -1225,17 +1228,60 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
     int r = check_grid_args(precision, num_w_planes, num_points, freq_per_chan, polarizations, dx,
                             dw, w_support, grid_size_x, grid_size_y);
     if (r) return fail(r);
-    if (!cube || !image_real || !image_imag) return fail(LOFAR_EARG);
-    {
-        CallGuard cg(stream);
-        if (cg.status) return cg.status;
-        r = degrid_transform(*cg.c, cg.st, precision, num_w_planes, grid_size_x, grid_size_y,
+    if (!cube || !image_real || !image_imag ||
+        (num_points > 0 && (!uu || !vv || !ww || !weight || !model_real || !model_imag)))
+        return fail(LOFAR_EARG);
+    CallGuard cg(stream);
+    if (cg.status) return cg.status;
+    DeviceCtx& c = *cg.c;
+    const hipStream_t st = cg.st;
+    const Geometry g = make_geometry(num_w_planes, w_support, grid_size_x, grid_size_y, dx, dw);
+    if (2 * g.K + 1 > kMaxTileWin) {
+        // wide windows: validated, transformed whole, interpolated per sample
+        r = validate_direct(c, nullptr, num_points, uu, vv, ww, g, st);
+        if (r) return r;
+        r = degrid_transform(c, st, precision, num_w_planes, grid_size_x, grid_size_y,
                              resolution, wmin, wmax, image_real, image_imag, cube);
         if (r) return r;
+        HIPCHK(stage(c, kStDegrid, st, [&] {
+            return degrid_direct(precision, num_points, freq_per_chan, polarizations, uu, vv, ww,
+                                 weight, g, cube, model_real, model_imag, st);
+        }));
+        return LOFAR_OK;
     }
-    return lofar_degrid_cube_dev(precision, num_w_planes, num_points, freq_per_chan, polarizations,
-                                 uu, vv, ww, weight, dx, dw, w_support, grid_size_x, grid_size_y,
-                                 cube, model_real, model_imag, stream);
+    // the home sort first (a bad sample is reported before anything is written), then
+    // the tiles it reads, the transforms storing only those, and the interpolation
+    Buckets b;
+    long long total = 0;
+    r = build_buckets(c, kBucketHome, num_points, uu, vv, ww, g, st, b, total);
+    if (r) return r;
+    const size_t ebytes = align256((size_t)total * sizeof(DEntry));
+    const size_t l2 = level2_layout(b, total, nullptr);
+    r = ensure(c, c.ents, 2 * ebytes + l2 + 256);
+    if (r) return r;
+    char* eb = (char*)c.ents.ptr;
+    DEntry* rec = (DEntry*)eb;
+    DEntry* ents = (DEntry*)(eb + ebytes);
+    level2_layout(b, total, eb + 2 * ebytes);
+    unsigned long long* need = nullptr;
+    HIPCHK(stage(c, kStScatter, st, [&] {
+        return bucket_sort_home(num_points, uu, vv, ww, polarizations == 1 ? weight : nullptr, g,
+                                b, rec, ents, st);
+    }));
+    if (pow2_ok(grid_size_x) && pow2_ok(grid_size_y)) {
+        r = ensure(c, c.occ, (size_t)need_words(g) * 8 + 256);
+        if (r) return r;
+        need = (unsigned long long*)c.occ.ptr;
+        HIPCHK(stage(c, kStScatter, st, [&] { return need_tiles(g, b, need, st); }));
+    }
+    r = degrid_transform(c, st, precision, num_w_planes, grid_size_x, grid_size_y, resolution,
+                         wmin, wmax, image_real, image_imag, cube, need);
+    if (r) return r;
+    HIPCHK(stage(c, kStDegrid, st, [&] {
+        return degrid_tiles(precision, g, b, ents, cube, weight, freq_per_chan, polarizations,
+                            model_real, model_imag, st);
+    }));
+    return LOFAR_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1352,13 +1398,19 @@ int lofar_plan_create(int precision, int num_w_planes, long num_points, long fre
                                 hb, (DEntry*)hbuf, (DEntry*)(hbuf + hbytes), st));
         const long long nbh = hb.map.nbuckets();
         const size_t hobytes = align256((size_t)(nbh + 1) * sizeof(long long));
-        r = alloc(&p->hmem, hobytes + hbytes + wbytes);
+        const bool pow2 = pow2_ok(grid_size_x) && pow2_ok(grid_size_y);
+        const size_t nbytes = pow2 ? align256((size_t)need_words(g) * 8) : 0;
+        r = alloc(&p->hmem, hobytes + hbytes + wbytes + nbytes);
         if (r) return r;
         p->hmap = hb.map;
         p->htotal = htotal;
         p->hoff = (long long*)p->hmem;
         p->hent = (DEntry*)(p->hmem + hobytes);
         p->wt = (float*)(p->hmem + hobytes + hbytes);
+        if (pow2) {
+            p->need = (unsigned long long*)(p->hmem + hobytes + hbytes + wbytes);
+            HIPCHK(need_tiles(g, hb, p->need, st));
+        }
         HIPCHK(hipMemcpyAsync(p->hoff, hb.offsets, (nbh + 1) * sizeof(long long),
                               hipMemcpyDeviceToDevice, st));
         if (htotal > 0)
@@ -1400,7 +1452,7 @@ int lofar_plan_degrid(lofar_plan plan, double resolution, double wmin, double wm
     if (cg.c->device != plan->device) return fail(LOFAR_EARG);
     DeviceCtx& c = *cg.c;
     int r = degrid_transform(c, cg.st, plan->precision, plan->nw, plan->nx, plan->ny, resolution,
-                             wmin, wmax, image_real, image_imag, cube);
+                             wmin, wmax, image_real, image_imag, cube, plan->need);
     if (r) return r;
     const Geometry g = make_geometry(plan->nw, plan->support, plan->nx, plan->ny, plan->dx,
                                      plan->dw);

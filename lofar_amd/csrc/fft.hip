@@ -76,7 +76,10 @@ __global__ __launch_bounds__((Plan<N, E>::TPL)) void k_fft_rows(const C2<Real>* 
 // W*8 bytes contiguous per row access), LDS laid out [slot][W].  Workgroups
 // are numbered so that the strips one XCD runs at a time are adjacent in
 // memory and share its L2 lines.
-template <typename Real, int N, int E, int W, int ROUNDS, int SIGN, bool OCC>
+// OCC: `occ` holds the forward path's occupancy bits (rows of empty tiles read as
+// zeros).  NEED: `occ` holds the tiles a degridding reads (same indexing); rows of
+// other tiles are not stored, and a strip whose tile column holds none is skipped.
+template <typename Real, int N, int E, int W, int ROUNDS, int SIGN, bool OCC, bool NEED = false>
 __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Real>* in,
                                                                  C2<Real>* out,
                                                                  const C2<Real>* tw, int nx,
@@ -117,6 +120,16 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
         (unsigned)((base - plane0) * (long long)sizeof(T2)) + (unsigned)t * row_step;
     // complex doubles: the stages' twiddles as powers of roots loaded here, ahead
     // of the column data
+    constexpr int NWD = ((N + kTile - 1) / kTile + 63) / 64;  // mask words per tile column
+    static_assert(NWD <= 2, "tile masks cover at most 128 tile rows");
+    unsigned long long need0 = ~0ull, need1 = ~0ull;
+    if constexpr (NEED) {
+        const int ntu = (nx + kTile - 1) / kTile;
+        const unsigned long long* nm = occ + (plane * ntu + u0 / kTile) * NWD;
+        need0 = nm[0];
+        need1 = NWD > 1 ? nm[1] : 0ull;
+        if ((need0 | need1) == 0ull) return;  // nothing of this tile column is read
+    }
     constexpr bool kRoots = sizeof(T2) >= 16 && W * P::TPL <= 512;  // not at the 128-VGPR cap
     constexpr int NRT = kRoots ? root_count<N, E>(1) : 0;
     T2 rt[NRT > 0 ? NRT : 1];
@@ -128,10 +141,8 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(in + plane0), (short)0, plane_bytes, 0x00020000);  // raw, range-checked
         // bit tv of word tv/64: tile row tv of this strip's tile column holds samples
-        constexpr int NWD = (N / kTile + 63) / 64;
-        static_assert(NWD <= 2, "occupancy reads cover at most 128 tile rows");
         const int ntu = (nx + kTile - 1) / kTile;
-        const unsigned long long* om = occ + (plane * ntu + u0 / kTile) * (NWD > 0 ? NWD : 1);
+        const unsigned long long* om = occ + (plane * ntu + u0 / kTile) * NWD;
         const unsigned long long live0 = om[0], live1 = NWD > 1 ? om[1] : 0ull;
         if ((live0 | live1) == 0ull) {
             // no samples anywhere in this strip's tile column: its transform is
@@ -166,6 +177,10 @@ __global__ __launch_bounds__((W * Plan<N, E>::TPL)) void k_fft_cols(const C2<Rea
             (void*)(out + plane0), (short)0, plane_bytes, 0x00020000);
 #pragma unroll
         for (int e = 0; e < P::E; e++) {
+            if constexpr (NEED) {
+                const int tv = (t + P::out_index(0, e)) / kTile;
+                if (!((((tv < 64) ? need0 : need1) >> (tv & 63)) & 1ull)) continue;
+            }
             T2 x = v[e];
             x.x *= scale;
             x.y *= scale;
@@ -472,9 +487,11 @@ static hipError_t launch_cols_cfg(int sign, const void* in, void* out, const voi
     const long long nstrips = (long long)strips * nw;
     // the kernel addresses a plane with 32-bit byte offsets
     if ((long long)N * nx * (long long)sizeof(T2) >= (1LL << 31)) return hipErrorInvalidValue;
+    // forward transforms read `occ` as occupancy, inverse ones as the tiles to store
     auto k = sign > 0 ? (occ ? k_fft_cols<Real, N, E, W, ROUNDS, 1, true>
                              : k_fft_cols<Real, N, E, W, ROUNDS, 1, false>)
-                      : k_fft_cols<Real, N, E, W, ROUNDS, -1, false>;
+                      : (occ ? k_fft_cols<Real, N, E, W, ROUNDS, -1, false, true>
+                             : k_fft_cols<Real, N, E, W, ROUNDS, -1, false>);
     // strip-major planes: whole strips of a power-of-two width that W divides
     if (sm_log >= 0 && ((1 << sm_log) % W != 0 || nx % (1 << sm_log) != 0))
         return hipErrorInvalidValue;

@@ -496,8 +496,9 @@ template <typename Real>
 hipError_t fft_rows(int sign, const void* in, void* out, const void* tw, int nx, long long lines,
                     double scale, hipStream_t st);
 template <typename Real>
-// occ (optional, forward transforms only): per-tile occupancy bits from
-// grid_tiles; rows of empty tiles are read as zeros
+// occ (optional): for forward transforms (sign +1), per-tile occupancy bits from
+// grid_tiles, rows of empty tiles read as zeros; for inverse transforms (sign -1),
+// the tiles to store (need_tiles), other rows left unwritten
 // sm_log >= 0: the planes are strip-major, [nx/L][ny][L] with L = 2^sm_log columns
 hipError_t fft_cols(int sign, const void* in, void* out, const void* tw, int nx, int ny, int nw,
                     double scale, hipStream_t st, const unsigned long long* occ = nullptr,

@@ -729,6 +729,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
         __syncthreads();
         for (int blk = tid; blk < kNBlk * kNBlk; blk += NT) {
             const unsigned long long b = bsum[blk];
+            if (b == 0ull) continue;  // no entry touches the block: it bounds nothing
             int ex = 0;
             frexp((double)(b & ((1ull << kCntShift) - 1)) * (norm * (double)m / (double)kBndUnit),
                   &ex);  // bound < 2^ex
@@ -1189,6 +1190,24 @@ __global__ __launch_bounds__(256) void k_degrid_direct(long long n, long long F,
     }
 }
 
+// Tiles a degridding reads: every tile holding home entries and its 8 neighbours (the
+// window's halo, K <= 8 cells, stays inside them), as bit tv%64 of word (w, tu, tv/64),
+// the indexing of grid_tiles' occupancy bits.  The words are cleared beforehand.
+__global__ __launch_bounds__(256) void k_need_tiles(Geometry g, TileMap tm,
+                                                    const long long* offsets,
+                                                    unsigned long long* need)
+{
+    const long long bkt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bkt >= tm.nbuckets()) return;
+    long long w;
+    int tv, tu;
+    if (!tm.decode(bkt, g, w, tv, tu) || offsets[bkt] == offsets[bkt + 1]) return;
+    const int nwd = (g.ntv + 63) / 64;
+    for (int u = max(tu - 1, 0); u <= min(tu + 1, g.ntu - 1); u++)
+        for (int v = max(tv - 1, 0); v <= min(tv + 1, g.ntv - 1); v++)
+            atomicOr(&need[(w * g.ntu + u) * nwd + v / 64], 1ull << (v % 64));
+}
+
 // ---------------------------------------------------------------------------
 // host dispatch
 
@@ -1251,6 +1270,20 @@ static void launch_grid(int precision, const Geometry& g, const Buckets& b, cons
         case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
         default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
     }
+}
+
+long long need_words(const Geometry& g) { return (long long)g.nw * g.ntu * ((g.ntv + 63) / 64); }
+
+hipError_t need_tiles(const Geometry& g, const Buckets& b, unsigned long long* need,
+                      hipStream_t st)
+{
+    hipError_t e = fill_u32((unsigned*)need, 2 * need_words(g), 0u, st);
+    if (e != hipSuccess) return e;
+    const long long nb = b.map.nbuckets();
+    if (nb > 0)
+        hipLaunchKernelGGL(k_need_tiles, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, g,
+                           b.map, b.offsets, need);
+    return hipGetLastError();
 }
 
 hipError_t validate_samples(long long n, const double* uu, const double* vv, const double* ww,

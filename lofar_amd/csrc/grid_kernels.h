@@ -199,10 +199,16 @@ hipError_t bucket_count(int mode, long long n, const double* uu, const double* v
 hipError_t bucket_scan(const Buckets& b, hipStream_t st);
 // Level 1 scatter into super-tile order (rec: one Entry64 per record), then level
 // 2 into (w, tile) order (entries: Entry64 for FP64, Entry32 for FP32C).
+// index_vis: the entries carry each sample's index in the real part's slot instead
+// of its visibility (a sort kept across calls; plan_fill puts visibilities in).
 hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P,
                             const double* uu, const double* vv, const double* ww, const float* vr,
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
-                            Entry64* rec, void* entries, hipStream_t st);
+                            Entry64* rec, void* entries, hipStream_t st, bool index_vis = false);
+// total entries of index_vis sorted entries (pent) with this call's visibilities, into ent
+hipError_t plan_fill(int precision, long long total, const void* pent, void* ent, long long F,
+                     long long P, const float* vr, const float* vi, const float* wt,
+                     hipStream_t st);
 // wt1 (one polarisation): the samples' weights, stored in the entries' spare word
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
                             const float* wt1, const Geometry& g, const Buckets& b, DEntry* rec,
@@ -229,6 +235,12 @@ hipError_t grid_direct(int precision, long long n, long long F, long long P, con
 hipError_t degrid_direct(int precision, long long n, long long F, long long P, const double* uu,
                          const double* vv, const double* ww, const float* wt, const Geometry& g,
                          const void* cube, void* model_re, void* model_im, hipStream_t st);
+// The tiles a degridding from home buckets b reads (each home tile and its 8
+// neighbours): need_words(g) words, grid_tiles' occupancy indexing; for the adjoint's
+// inverse column transform (fft_cols with sign -1), which then stores only those.
+long long need_words(const Geometry& g);
+hipError_t need_tiles(const Geometry& g, const Buckets& b, unsigned long long* need,
+                      hipStream_t st);
 hipError_t degrid_tiles(int precision, const Geometry& g, const Buckets& b, const DEntry* entries,
                         const void* cube, const float* weight, long long F, long long P,
                         void* model_re, void* model_im, hipStream_t st);

@@ -284,6 +284,76 @@ def degrid(image_real, image_imag, uu, vv, ww, weight, num_w_planes, w_support, 
     return mr, mi
 
 
+class Plan:
+    """One sort of fixed samples (lofar_plan_create) for repeated ``image`` and
+    ``degrid`` calls with new visibilities or a new image: the same results as the
+    unplanned calls, without re-sorting or host synchronisation.  The plan owns
+    device memory (released by ``close`` or when the object is collected); the
+    degridding weights are fixed at creation, the imaging weights come with each call."""
+
+    def __init__(self, uu, vv, ww, weight, num_w_planes, w_support, grid_size_x, grid_size_y,
+                 precision="fp64", freq_per_chan=1, polarizations=1):
+        torch = _torch()
+        self.precision = _prec(precision)
+        self.nw, self.support = num_w_planes, w_support
+        self.nx, self.ny = grid_size_x, grid_size_y
+        self.n, self.F, self.P = uu.numel(), freq_per_chan, polarizations
+        self.device = uu.device
+        h = C.c_void_p()
+        check("lofar_plan_create", lib().lofar_plan_create(
+            self.precision, num_w_planes, self.n, freq_per_chan, polarizations,
+            _ptr(uu, torch.float64, "uu"), _ptr(vv, torch.float64, "vv"),
+            _ptr(ww, torch.float64, "ww"), _ptr(weight, torch.float32, "weight"),
+            1.0 / grid_size_x, 1.0 / num_w_planes, w_support, grid_size_x, grid_size_y,
+            _stream(), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().lofar_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def image(self, vis_real, vis_img, weight, resolution, wmin, wmax, cube=None,
+              image_real=None, image_imag=None, accumulate=False):
+        """``image`` on the planned samples; returns (image_real, image_imag)."""
+        torch = _torch()
+        if cube is None:
+            cube = torch.empty((self.nw, self.ny, self.nx), dtype=cube_dtype(self.precision),
+                               device=self.device)
+        cp = _check_cube(cube, self.precision, self.nw, self.ny, self.nx)
+        if image_real is None:
+            image_real = torch.empty((self.ny, self.nx), dtype=torch.float64, device=self.device)
+            image_imag = torch.empty((self.ny, self.nx), dtype=torch.float64, device=self.device)
+            accumulate = False
+        check("lofar_plan_image", lib().lofar_plan_image(
+            self._h, _ptr(vis_real, torch.float32, "vis_real"),
+            _ptr(vis_img, torch.float32, "vis_img"), _ptr(weight, torch.float32, "weight"),
+            resolution, wmin, wmax, cp, _ptr(image_real, torch.float64, "image_real"),
+            _ptr(image_imag, torch.float64, "image_imag"), int(accumulate), _stream()))
+        return image_real, image_imag
+
+    def degrid(self, image_real, image_imag, resolution, wmin, wmax, cube=None, model_real=None,
+               model_imag=None):
+        """``degrid`` on the planned samples; returns (model_real, model_imag)."""
+        torch = _torch()
+        if cube is None:
+            cube = torch.empty((self.nw, self.ny, self.nx), dtype=cube_dtype(self.precision),
+                               device=self.device)
+        cp = _check_cube(cube, self.precision, self.nw, self.ny, self.nx)
+        mr, mi, pr, pi = _model_out(self.n * self.F * self.P, self.precision, self.device,
+                                    model_real, model_imag)
+        check("lofar_plan_degrid", lib().lofar_plan_degrid(
+            self._h, resolution, wmin, wmax, _ptr(image_real, torch.float64, "image_real"),
+            _ptr(image_imag, torch.float64, "image_imag"), cp, pr, pi, _stream()))
+        return mr, mi
+
+
 # ---------------------------------------------------------------------------
 # per-stage kernel timing (HIP events on the stream the kernels run on)
 
