@@ -470,20 +470,24 @@ def test_image_pipeline_matches_oracle(cuda, n, nx, nw, support):
     assert e32 < 1e-6
 
 
+@pytest.mark.parametrize("n,nx,nw", [(3000, 512, 8), (40000, 2048, 2)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-6)])
-def test_image_ignores_stale_scratch_cube(cuda, precision, tol):
-    """The fused path may leave tiles without samples unwritten (their rows are read as
-    zeros by the column transform): stale NaNs in the caller's scratch cube must not leak
-    into the image.  Sparse coverage leaves most tiles of most planes empty."""
+def test_image_ignores_stale_scratch_cube(cuda, precision, tol, n, nx, nw):
+    """The fused path leaves tiles without samples unwritten (the column transform reads
+    them as zeros) and whole tile columns without samples unwritten after the column
+    transform (the row pass reads them as zeros): stale NaNs in the caller's scratch cube
+    must not leak into the image.  Sparse coverage leaves most tiles of most planes empty;
+    at 2048 the disk of samples leaves edge tile columns empty, one of them paired with a
+    column holding samples in a complex-float prefetch run."""
     import torch
     from lofar_amd import imager
-    d = synth(3000, 512, 8, seed=77)
-    want = O.image(8, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
-                   512, 512, d["resolution"], d["wmin"], d["wmax"])
-    cube = torch.full((8, 512, 512), float("nan"), dtype=imager.cube_dtype(precision),
+    d = synth(n, nx, nw, seed=77)
+    want = O.image(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"], 7,
+                   nx, nx, d["resolution"], d["wmin"], d["wmax"])
+    cube = torch.full((nw, nx, nx), float("nan"), dtype=imager.cube_dtype(precision),
                       device=cuda)
     for _ in range(2):
-        r, i = imager.image(*dev_inputs(d, cuda), 8, 7, 512, 512, d["resolution"], d["wmin"],
+        r, i = imager.image(*dev_inputs(d, cuda), nw, 7, nx, nx, d["resolution"], d["wmin"],
                             d["wmax"], precision=precision, cube=cube)
         assert rel(H(r) + 1j * H(i), want) < tol
 
