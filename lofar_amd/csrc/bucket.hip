@@ -19,6 +19,14 @@
 // every record into one entry per tile of its window (w-stacking_omp.c:93-96).
 // Entries are self-contained (no side arrays), so every scattered write is one
 // aligned 16- or 32-byte store.
+//
+// Both scatters run staged where their bins fit one per thread (level 1 up to
+// kStageMaxBins super-tiles, level 2 always): a round's records are counted, scanned
+// and placed in an LDS buffer in bin order, then stored with consecutive lanes on
+// consecutive slots, so each bin's records leave as a run instead of one lane per bin
+// (the scattered 16-byte stores cost two 32-byte writes each).
+#include <type_traits>
+
 #include "grid_kernels.h"
 
 namespace lofar {
@@ -179,6 +187,26 @@ __global__ __launch_bounds__(256) void k_scan_add(long long* out, const long lon
 // ---------------------------------------------------------------------------
 // level 1: scatter records into super-tile order
 
+// The record of sample i (position e.pu, e.pv and summed visibility e.vr, e.vi) in
+// super-tile (sv, su); IDX: the sample's index in the real part's slot instead.
+template <typename Rec, bool IDX>
+__device__ __forceinline__ Rec make_record(const Entry64& e, const TileMap& tm, int sv, int su,
+                                           long long i)
+{
+    if constexpr (sizeof(Rec) == sizeof(Rec32)) {
+        Rec32 r;
+        r.u = rec_axis(e.pu, (su << tm.lsst) * kTile, tm.rfb);
+        r.v = rec_axis(e.pv, (sv << tm.lsst) * kTile, tm.rfb);
+        r.vr = IDX ? __uint_as_float((unsigned)i) : (float)e.vr;
+        r.vi = (float)e.vi;
+        return r;
+    } else {
+        Entry64 r = e;
+        if constexpr (IDX) r.vr = __longlong_as_double(i);
+        return r;
+    }
+}
+
 // Rec: Entry64 (FP64 mode) or Rec32 (FP32 mode, position relative to the
 // super-tile, rec_axis)
 // GLOBAL: base1 is one global cursor per super-tile (zeroed), taken with global atomics.
@@ -230,20 +258,168 @@ __global__ __launch_bounds__(256) void k_l1_scatter_grid(long long n, long long 
                 for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
                     const long long k = tm.key_st(s[u].gw, sv, su);
                     const long long at = off1[k] + atomicAdd(&cur[k], 1u);
-                    if constexpr (sizeof(Rec) == sizeof(Rec32)) {
-                        Rec32 r;
-                        r.u = rec_axis(e[u].pu, (su << tm.lsst) * kTile, tm.rfb);
-                        r.v = rec_axis(e[u].pv, (sv << tm.lsst) * kTile, tm.rfb);
-                        r.vr = IDX ? __uint_as_float((unsigned)(i0 + u * 256)) : (float)e[u].vr;
-                        r.vi = (float)e[u].vi;
-                        rec[at] = r;
+                    rec[at] = make_record<Rec, IDX>(e[u], tm, sv, su, i0 + u * 256);
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// staged scatters: a 512-thread workgroup takes its items in rounds, counts the
+// round's records per bin in LDS, scans, places them in an LDS buffer in bin order
+// and then stores the buffer with consecutive lanes on consecutive slots of a bin,
+// so that a bin's records leave in runs instead of one lane per bin.  A round's
+// records beyond the buffer are stored directly (same slots).  One bin per thread.
+
+constexpr int kStageThreads = 512;  // two workgroups per CU overlap their rounds' phases
+constexpr int kStageBytes = 65536;  // LDS buffer of one round's entries (level 2)
+static_assert(kStageMaxBins <= kStageThreads && kStageMaxBins <= 65536, "one bin per thread");
+
+// Level 1: buffer capacity (records) and samples per thread and round, so that
+// buffer, keys and bins stay within half a CU's 160 KiB of LDS (two workgroups per
+// CU) and a round's records (about 1.02 per sample on 16x16-tile super-tiles)
+// mostly fit.
+template <typename Rec>
+struct L1Stage {
+    static constexpr int CAP = sizeof(Rec) == sizeof(Rec32) ? 4032 : 2144;
+    static constexpr int U = 4;
+    static_assert(CAP * (sizeof(Rec) + 2) + kStageMaxBins * 16 + 64 <= 81920, "LDS");
+};
+
+// Exclusive scan of one value per thread over the workgroup; the sum to total.
+// Ends with every wave past the barrier that publishes the wave sums.
+template <int NT>
+__device__ __forceinline__ unsigned block_exscan(unsigned v, unsigned* wsum, unsigned& total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    unsigned pre = 0u, tot = 0u;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        const unsigned s = wsum[w];
+        pre += w < wid ? s : 0u;
+        tot += s;
+    }
+    total = tot;
+    return pre + x - v;
+}
+
+// Per-round bin bookkeeping of a staged scatter.  gnext (thread tid's register, bin
+// tid): global slot of the bin's next record.  After open(): dst[k] + p is the
+// global slot of staged position p of bin k, pos[k] the bin's first staged position.
+template <int NT>
+struct StageBins {
+    long long* dst;
+    unsigned *cnt, *pos, *wsum;
+    // after the counting pass (and a barrier): the round's layout; zeroes the counts
+    __device__ __forceinline__ unsigned open(long long nbins, long long& gnext)
+    {
+        const int tid = threadIdx.x;
+        const unsigned c = tid < nbins ? cnt[tid] : 0u;
+        unsigned total;
+        const unsigned ex = block_exscan<NT>(c, wsum, total);
+        if (tid < nbins) {
+            dst[tid] = gnext - (long long)ex;
+            pos[tid] = ex;
+            cnt[tid] = 0u;
+            gnext += c;
+        }
+        __syncthreads();
+        return total;
+    }
+};
+
+template <typename Rec, bool IDX>
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_l1_scatter_staged(
+    long long n, long long F, long long P, const double* uu, const double* vv, const double* ww,
+    const float* vr, const float* vi, const float* wt, Geometry g, TileMap tm, long long chunk,
+    const long long* off1, const unsigned* base1, Rec* rec)
+{
+    constexpr int NT = kStageThreads;
+    constexpr int CAP = L1Stage<Rec>::CAP;
+    constexpr int U = L1Stage<Rec>::U;  // samples per thread and round
+    // visibility sum as the record keeps it (float for Rec32: the same bits as rounding later)
+    using V = typename std::conditional<sizeof(Rec) == sizeof(Rec32), float, double>::type;
+    __shared__ Rec stage[CAP];
+    __shared__ unsigned short skey[CAP];
+    __shared__ long long dst[kStageMaxBins];
+    __shared__ unsigned cnt[kStageMaxBins], pos[kStageMaxBins], wsum[NT / 64];
+    StageBins<NT> sb{dst, cnt, pos, wsum};
+    const int tid = threadIdx.x;
+    long long gnext = 0;
+    if (tid < tm.nk1) {
+        gnext = off1[tid] + base1[(long long)blockIdx.x * tm.nk1 + tid];
+        cnt[tid] = 0u;
+    }
+    __syncthreads();
+    const long long beg = (long long)blockIdx.x * chunk;
+    const long long end = min(n, beg + chunk);
+    for (long long r0 = beg; r0 < end; r0 += (long long)NT * U) {
+        // per sample only the position, plane (-1: no record) and visibility; the
+        // window is re-derived from the position in each pass
+        double pu[U], pv[U];
+        int gw[U];
+        V sr[U], si[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long i0 = r0 + (long long)u * NT + tid;
+            const long long i = min(i0, end - 1);
+            SampleWin s;
+            const bool ok = i0 < end && sample_window(uu[i], vv[i], ww[i], g, s) &&
+                            s.j0 <= s.j1 && s.k0 <= s.k1;
+            pu[u] = ok ? s.pu : 0.0;
+            pv[u] = ok ? s.pv : 0.0;
+            gw[u] = ok ? s.gw : -1;
+            if constexpr (IDX) {
+                sr[u] = si[u] = (V)0;
+            } else {
+                double a, b;
+                sample_vis(i, F, P, vr, vi, wt, a, b);
+                sr[u] = (V)a;
+                si[u] = (V)b;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (gw[u] < 0) continue;
+            const TileSpan t = tiles_of(window_of(pu[u], pv[u], g));
+            for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++)
+                for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++)
+                    atomicAdd(&cnt[tm.key_st(gw[u], sv, su)], 1u);
+        }
+        __syncthreads();
+        const unsigned total = sb.open(tm.nk1, gnext);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (gw[u] < 0) continue;
+            const TileSpan t = tiles_of(window_of(pu[u], pv[u], g));
+            const Entry64 e{pu[u], pv[u], (double)sr[u], (double)si[u]};
+            for (int sv = t.tv_a >> tm.lsst; sv <= t.tv_b >> tm.lsst; sv++)
+                for (int su = t.tu_a >> tm.lsst; su <= t.tu_b >> tm.lsst; su++) {
+                    const int k = (int)tm.key_st(gw[u], sv, su);
+                    const unsigned p = atomicAdd(&pos[k], 1u);
+                    const Rec r = make_record<Rec, IDX>(e, tm, sv, su,
+                                                        r0 + (long long)u * NT + tid);
+                    if (p < (unsigned)CAP) {
+                        stage[p] = r;
+                        skey[p] = (unsigned short)k;
                     } else {
-                        Entry64 r = e[u];
-                        if constexpr (IDX) r.vr = __longlong_as_double(i0 + u * 256);
-                        rec[at] = r;
+                        rec[dst[k] + p] = r;
                     }
                 }
         }
+        __syncthreads();
+        const unsigned m = min(total, (unsigned)CAP);
+        for (unsigned j = tid; j < m; j += NT) rec[dst[skey[j]] + j] = stage[j];
+        __syncthreads();
     }
 }
 
@@ -426,6 +602,27 @@ __global__ __launch_bounds__(256) void k_l2_scan(TileMap tm, const long long* eo
     if (k1 == 0 && tid == 0) offsets[tm.nk1 * nf] = eoff1[tm.nk1];
 }
 
+// The entry of record x (window s) for fine tile (fv, fu) of super-tile (sv, su).
+template <int OUT, typename Rec, typename Out>
+__device__ __forceinline__ Out make_entry(const Rec& x, const Window& s, const TileMap& tm, int sv,
+                                          int su, int fv, int fu)
+{
+    if constexpr (OUT == kOutFp32) {
+        const int u0 = ((su << tm.lsst) + fu) * kTile;
+        const int v0 = ((sv << tm.lsst) + fv) * kTile;
+        double pu, pv;
+        record_pos(x, tm, sv, su, pu, pv);
+        Entry32 e;
+        e.wu = pack_cell(s.gu - u0, pu - (double)s.gu);
+        e.wv = pack_cell(s.gv - v0, pv - (double)s.gv);
+        e.vr = x.vr;  // bit for bit (a plan's index, see k_l1_scatter_grid)
+        e.vi = x.vi;
+        return e;
+    } else {
+        return x;
+    }
+}
+
 template <int OUT, typename Rec, typename Out>
 __global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, const long long* eoff1,
                                                     const long long* nblocks, const L2Block* blk,
@@ -457,23 +654,82 @@ __global__ __launch_bounds__(256) void k_l2_scatter(Geometry g, TileMap tm, cons
         for (int fv = fv_a; fv <= fv_b; fv++) {
             for (int fu = fu_a; fu <= fu_b; fu++) {
                 const long long p = beg + atomicAdd(&cur[(fv << tm.lsst) | fu], 1u);
-                if constexpr (OUT == kOutFp32) {
-                    const int u0 = ((su << tm.lsst) + fu) * kTile;
-                    const int v0 = ((sv << tm.lsst) + fv) * kTile;
-                    double pu, pv;
-                    record_pos(x, tm, sv, su, pu, pv);
-                    Entry32 e;
-                    e.wu = pack_cell(s.gu - u0, pu - (double)s.gu);
-                    e.wv = pack_cell(s.gv - v0, pv - (double)s.gv);
-                    e.vr = x.vr;  // bit for bit (a plan's index, see k_l1_scatter_grid)
-                    e.vi = x.vi;
-                    ent[p] = e;
-                } else {
-                    ent[p] = x;
-                }
+                ent[p] = make_entry<OUT, Rec, Out>(x, s, tm, sv, su, fv, fu);
             }
         }
       }
+    }
+}
+
+// Level-2 scatter of one block, staged (see k_l1_scatter_staged): rounds of U records
+// per thread, the round's entries placed in LDS in fine-tile order, then stored in runs.
+template <int OUT, typename Rec, typename Out>
+__global__ __launch_bounds__(kStageThreads) void k_l2_scatter_staged(
+    Geometry g, TileMap tm, const long long* eoff1, const long long* nblocks, const L2Block* blk,
+    const unsigned* cnt2, const Rec* rec, Out* ent)
+{
+    constexpr int NT = kStageThreads;
+    constexpr int CAP = kStageBytes / (int)sizeof(Out);
+    constexpr int U = sizeof(Out) == sizeof(Entry32) ? 4 : 2;  // records per thread and round
+    __shared__ Out stage[CAP];
+    __shared__ unsigned char skey[CAP];
+    __shared__ long long dst[kMaxFine];
+    __shared__ unsigned cnt[kMaxFine], pos[kMaxFine], wsum[NT / 64];
+    StageBins<NT> sb{dst, cnt, pos, wsum};
+    const long long b = blockIdx.x;
+    if (b >= *nblocks) return;  // the grid is sized from an upper bound
+    const int tid = threadIdx.x;
+    const int nf = tm.sst * tm.sst;
+    const L2Block d = blk[b];
+    int sv, su;
+    tm.super_of(d.k1, sv, su);
+    long long gnext = 0;
+    if (tid < nf) {
+        gnext = eoff1[d.k1] + cnt2[b * nf + tid];
+        cnt[tid] = 0u;
+    }
+    __syncthreads();
+    for (long long r0 = d.beg; r0 < d.end; r0 += (long long)NT * U) {
+        Rec xs[U];
+        Window s[U];
+        int fu_a[U], fu_b[U], fv_a[U], fv_b[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long r = r0 + (long long)u * NT + tid;
+            xs[u] = rec[min(r, d.end - 1)];
+            ok[u] = r < d.end && record_tiles<OUT>(xs[u], g, tm, sv, su, fu_a[u], fu_b[u],
+                                                   fv_a[u], fv_b[u], s[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;
+            for (int fv = fv_a[u]; fv <= fv_b[u]; fv++)
+                for (int fu = fu_a[u]; fu <= fu_b[u]; fu++)
+                    atomicAdd(&cnt[(fv << tm.lsst) | fu], 1u);
+        }
+        __syncthreads();
+        const unsigned total = sb.open(nf, gnext);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;
+            for (int fv = fv_a[u]; fv <= fv_b[u]; fv++)
+                for (int fu = fu_a[u]; fu <= fu_b[u]; fu++) {
+                    const int f = (fv << tm.lsst) | fu;
+                    const unsigned p = atomicAdd(&pos[f], 1u);
+                    const Out e = make_entry<OUT, Rec, Out>(xs[u], s[u], tm, sv, su, fv, fu);
+                    if (p < (unsigned)CAP) {
+                        stage[p] = e;
+                        skey[p] = (unsigned char)f;
+                    } else {
+                        ent[dst[f] + p] = e;
+                    }
+                }
+        }
+        __syncthreads();
+        const unsigned m = min(total, (unsigned)CAP);
+        for (unsigned j = tid; j < m; j += NT) ent[dst[skey[j]] + j] = stage[j];
+        __syncthreads();
     }
 }
 
@@ -539,8 +795,12 @@ static hipError_t level2(const Geometry& g, const Buckets& b, const Rec* rec, Ou
                        b.l2_blk, rec, b.l2_cnt);
     hipLaunchKernelGGL(k_l2_scan, dim3((unsigned)nk1), dim3(256), 0, st, b.map, b.eoff1, b.l2_boff,
                        b.l2_cnt, b.offsets);
-    hipLaunchKernelGGL((k_l2_scatter<OUT, Rec, Out>), dim3(grid), dim3(256), 0, st, g, b.map,
-                       b.eoff1, nblocks, b.l2_blk, b.l2_cnt, rec, ent);
+    if (b.stage & kStageL2)
+        hipLaunchKernelGGL((k_l2_scatter_staged<OUT, Rec, Out>), dim3(grid), dim3(kStageThreads),
+                           0, st, g, b.map, b.eoff1, nblocks, b.l2_blk, b.l2_cnt, rec, ent);
+    else
+        hipLaunchKernelGGL((k_l2_scatter<OUT, Rec, Out>), dim3(grid), dim3(256), 0, st, g, b.map,
+                           b.eoff1, nblocks, b.l2_blk, b.l2_cnt, rec, ent);
     return hipGetLastError();
 }
 
@@ -551,6 +811,12 @@ static void launch_l1_scatter_grid(bool global_l1, bool index_vis, size_t lds, h
                                    const float* vi, const float* wt, const Geometry& g,
                                    const Buckets& b, Rec* rec)
 {
+    if ((b.stage & kStageL1) && !global_l1) {
+        auto k = index_vis ? k_l1_scatter_staged<Rec, true> : k_l1_scatter_staged<Rec, false>;
+        hipLaunchKernelGGL(k, dim3((unsigned)b.nwg), dim3(kStageThreads), 0, st, n, F, P, uu, vv,
+                           ww, vr, vi, wt, g, b.map, b.chunk, b.off1, b.base1, rec);
+        return;
+    }
     auto k = global_l1 ? (index_vis ? k_l1_scatter_grid<Rec, true, true>
                                     : k_l1_scatter_grid<Rec, true, false>)
                        : (index_vis ? k_l1_scatter_grid<Rec, false, true>

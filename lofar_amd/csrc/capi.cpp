@@ -520,29 +520,45 @@ Geometry make_geometry(int nw, int support, int nx, int ny, double dx, double dw
     return g;
 }
 
-// Bucket-sort tuning: super-tile side (tiles), samples per level-1 workgroup
-// and records per level-2 block.  LOFAR_BUCKET_CFG="sst,chunk,block" overrides,
-// for measurement.
+// Bucket-sort tuning: super-tile side (tiles), samples per level-1 workgroup,
+// records per level-2 block, and which scatters are staged through LDS (kStageL1 |
+// kStageL2).  LOFAR_BUCKET_CFG="sst,chunk,block" and LOFAR_BUCKET_STAGE=<bits>
+// override, for measurement.  16x16-tile super-tiles keep a 4096-record round of the
+// staged level 1 at 8 records per super-tile on a 4096^2 x 32 cube.
 struct BucketCfg {
     int sst;
     long long chunk, block;
+    int stage;
 };
 
 BucketCfg bucket_cfg()
 {
-    static BucketCfg cfg{0, 0, 0};
-    if (!cfg.sst) {
-        int a = 8;
+    static const BucketCfg cfg = [] {
+        int stage = kStageL1 | kStageL2;
+        if (const char* e = getenv("LOFAR_BUCKET_STAGE")) stage = atoi(e) & (kStageL1 | kStageL2);
+        int a = (stage & kStageL1) ? 16 : 8;
         long long b = 16384, l2 = 8192;
         if (const char* e = getenv("LOFAR_BUCKET_CFG")) sscanf(e, "%d,%lld,%lld", &a, &b, &l2);
         if (a < 1 || a > 16 || (a & (a - 1))) a = 16;
         if (b < 256) b = 256;
         if (l2 < 256) l2 = 256;
-        cfg.chunk = b;
-        cfg.block = l2;
-        cfg.sst = a;
-    }
+        return BucketCfg{a, b, l2, stage};
+    }();
     return cfg;
+}
+
+// Launch order of the per-tile kernels (TileMap::launch_bucket): blocks of 2^olog x
+// 2^olog tiles, 0 for bucket order.  LOFAR_TILE_ORDER=<olog> overrides, for measurement.
+int tile_order_log(const TileMap& m)
+{
+    static const int want = [] {
+        const char* e = getenv("LOFAR_TILE_ORDER");
+        const int v = e ? atoi(e) : 3;
+        return v < 0 || v > 6 ? 3 : v;
+    }();
+    const int side = 1 << want;
+    if (want == 0 || ((m.nstu << m.lsst) % side) || ((m.nstv << m.lsst) % side)) return 0;
+    return want;
 }
 
 // Level-2 block table and fine counts, placed after the records once the entry
@@ -602,9 +618,11 @@ int build_buckets(DeviceCtx& c, int mode, long n, const double* uu, const double
     m.nstv = (g.ntv + sst - 1) / sst;
     m.rfb = rec_fraction_bits(sst);
     m.nk1 = super_tiles(sst);
+    m.olog = tile_order_log(m);
     // beyond the LDS histograms' capacity, level 1 counts and places records with global
     // atomics on per-super-tile counters (slower, no size limit)
     b.global_l1 = m.nk1 > l1_lds_max();
+    b.stage = cfg.stage & ((b.global_l1 || m.nk1 > kStageMaxBins) ? kStageL2 : ~0);
     b.nwg = n > 0 ? (n + chunk - 1) / chunk : 1;
     if (b.nwg > 4096) b.nwg = 4096;
     b.chunk = n > 0 ? (n + b.nwg - 1) / b.nwg : 1;

@@ -111,10 +111,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles(Geometry g, TileMap tm, const
                                                     int sm_log)
 {
     __shared__ double acc[2 * kTile * kTile];
-    const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
-    if (!tm.decode(bkt, g, w, tv, tu)) return;
+    const long long bkt = tm.launch_bucket(blockIdx.x, g, w, tv, tu);
+    if (bkt < 0) return;
     const int u0 = tu * kTile, v0 = tv * kTile;
     const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
@@ -312,10 +312,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx(Geometry g, TileMap tm, co
     __shared__ float wred[NT / 64];
     __shared__ double wsum[NT / 64];
     __shared__ int wfin[NT / 64];
-    const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
-    if (!tm.decode(bkt, g, w, tv, tu)) return;
+    const long long bkt = tm.launch_bucket(blockIdx.x, g, w, tv, tu);
+    if (bkt < 0) return;
     const int u0 = tu * kTile, v0 = tv * kTile;
     const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
@@ -644,10 +644,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
     __shared__ double wsum[NT / 64];
     __shared__ int wfin[NT / 64];
     __shared__ int dense_blk, tile_ex;
-    const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
-    if (!tm.decode(bkt, g, w, tv, tu)) return;
+    const long long bkt = tm.launch_bucket(blockIdx.x, g, w, tv, tu);
+    if (bkt < 0) return;
     const int u0 = tu * kTile, v0 = tv * kTile;
     const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
@@ -910,10 +910,10 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_fx64(Geometry g, TileMap tm,
     __shared__ double wred[NT / 64];
     __shared__ double wsum[NT / 64];
     __shared__ int wfin[NT / 64];
-    const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
-    if (!tm.decode(bkt, g, w, tv, tu)) return;
+    const long long bkt = tm.launch_bucket(blockIdx.x, g, w, tv, tu);
+    if (bkt < 0) return;
     const int u0 = tu * kTile, v0 = tv * kTile;
     const int tcols = min(kTile, g.nx - u0), trows = min(kTile, g.ny - v0);
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
@@ -1035,10 +1035,10 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
     constexpr int H = (WIN - 1) / 2;
     constexpr int RS = kTile + 2 * H;
     __shared__ T2 region[RS * RS];
-    const long long bkt = blockIdx.x;
     long long w;
     int tv, tu;
-    if (!tm.decode(bkt, g, w, tv, tu)) return;
+    const long long bkt = tm.launch_bucket(blockIdx.x, g, w, tv, tu);
+    if (bkt < 0) return;
     const int u0 = tu * kTile, v0 = tv * kTile;
     const long long beg = offsets[bkt], end = offsets[bkt + 1];
     if (beg == end) return;

@@ -83,6 +83,7 @@ struct TileMap {
     int sst, lsst;   // super-tile side in tiles (power of two) and its log2
     int nstu, nstv;  // super-tiles along u and v
     int rfb;         // fraction bits of an Rec32 position (rec_fraction_bits(sst))
+    int olog;        // launch order of the per-tile kernels (launch_bucket)
     long long nk1;   // super-tile buckets, nw*nstv*nstu
     __host__ __device__ long long key1(int w, int tv, int tu) const
     {
@@ -115,6 +116,27 @@ struct TileMap {
         tv = (stv << lsst) + (f >> lsst);
         tu = (stu << lsst) + (f & (sst - 1));
         return tv < g.ntv && tu < g.ntu;
+    }
+    // Launch order of the per-tile kernels (one workgroup per bucket slot): with olog > 0,
+    // workgroup x takes the tile at x of a walk through each plane in blocks of
+    // 2^olog x 2^olog tiles (row-major blocks, row-major tiles inside), so that the
+    // tiles in flight together do not depend on the super-tile side; olog = 0: bucket
+    // order.  Returns that tile's bucket, or -1 for the slots of partial super-tiles
+    // beyond the grid.
+    __device__ long long launch_bucket(long long x, const Geometry& g, long long& w, int& tv,
+                                       int& tu) const
+    {
+        if (olog == 0) return decode(x, g, w, tv, tu) ? x : -1;
+        const long long per_plane = (long long)nstv * nstu << (2 * lsst);
+        w = x / per_plane;
+        const long long r = x - w * per_plane;
+        const long long ob = r >> (2 * olog);  // block of the padded plane
+        const int nou = (nstu << lsst) >> olog;
+        const int f = (int)(r & ((1 << (2 * olog)) - 1));
+        tv = (int)(ob / nou << olog) + (f >> olog);
+        tu = (int)(ob % nou << olog) + (f & ((1 << olog) - 1));
+        if (tv >= g.ntv || tu >= g.ntu) return -1;
+        return (key1((int)w, tv, tu) << (2 * lsst)) | fine(tv, tu);
     }
 };
 
@@ -171,6 +193,7 @@ struct Buckets {
     long long nwg, chunk;  // level-1 workgroups and samples per workgroup
     bool global_l1;        // level-1 histograms and cursors in global memory (more
                            // super-tiles than the LDS histograms hold)
+    int stage;             // kStageL1 | kStageL2: scatters staged through LDS
     unsigned* count1;      // [nk1] records per super-tile
     long long* off1;       // [nk1+1] record offsets
     unsigned* count1e;     // [nk1] entries per super-tile
@@ -192,6 +215,10 @@ struct Buckets {
 };
 
 enum { kBucketFootprint = 0, kBucketHome = 1 };
+// Staged scatters (LDS buffer, records stored in runs): level 1 takes up to
+// kStageMaxBins super-tiles (LDS histograms), level 2 any block.
+enum { kStageL1 = 1, kStageL2 = 2 };
+constexpr long long kStageMaxBins = 512;
 
 // Level 1 (bucket.hip): count records and entries per super-tile, validate.
 hipError_t bucket_count(int mode, long long n, const double* uu, const double* vv, const double* ww,

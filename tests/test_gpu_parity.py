@@ -63,6 +63,21 @@ def random_case(seed, n, nx, ny, nw, F=1, P=1, edge=True, nan=0.02):
     return dict(uu=u, vv=v, ww=w, vis_real=vr, vis_img=vi, weight=wt)
 
 
+def corner_case(seed, n, nx, nw):
+    """Every sample within two cells of an interior corner of the 64x64 tiles (nx x nx
+    grid): a 7-cell window then covers 4 tiles, so each record expands into 4 entries (and
+    into 4 records when every tile is a super-tile)."""
+    rng = np.random.default_rng(seed)
+    t = nx // 64
+    u = (64.0 * rng.integers(1, t, n) + rng.uniform(-2.0, 2.0, n)) / nx
+    v = (64.0 * rng.integers(1, t, n) + rng.uniform(-2.0, 2.0, n)) / nx
+    w = rng.uniform(0.0, 1 - 1e-12, n)
+    vr = rng.standard_normal(n).astype(np.float32)
+    vi = rng.standard_normal(n).astype(np.float32)
+    wt = rng.uniform(0.2, 2.0, n).astype(np.float32)
+    return dict(uu=u, vv=v, ww=w, vis_real=vr, vis_img=vi, weight=wt)
+
+
 # ---------------------------------------------------------------------------
 # gridding
 
@@ -207,10 +222,11 @@ import sys, numpy as np, torch
 sys.path.insert(0, sys.argv[1])
 from oracle import oracle as O
 from lofar_amd import imager
-from tests.test_gpu_parity import random_case, dev_inputs, rel, H
+from tests.test_gpu_parity import corner_case, random_case, dev_inputs, rel, H
 dev = torch.device("cuda:0")
-for seed, n, nx, ny, nw in ((21, 40000, 300, 200, 3), (22, 3000, 1100, 64, 2)):
-    d = random_case(seed, n, nx, ny, nw)
+for seed, n, nx, ny, nw in ((21, 40000, 300, 200, 3), (22, 3000, 1100, 64, 2),
+                            (23, 30000, 512, 512, 2)):
+    d = random_case(seed, n, nx, ny, nw) if seed != 23 else corner_case(seed, n, nx, nw)
     st, want = O.wstack(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
                         1 / nx, 1 / nw, 7, nx, ny)
     assert st == O.OK
@@ -226,20 +242,46 @@ print("BUCKETS_OK")
 
 
 @pytest.mark.parametrize("cfg", ["1,256,256", "2,1000,300", "16,300,100000", "8,1000000,8192",
-                                 "global"])
+                                 "1,100000,256", "global", "stage=0", "stage=1", "stage=2"])
 def test_bucket_sort_configurations(cuda, cfg):
     """Other super-tile sides, level-1 chunk and level-2 block sizes (LOFAR_BUCKET_CFG, read once per
     process) give the same cube; "global": level 1 with global histograms and cursors (the
     path for more super-tiles than the LDS histograms hold, forced here with
-    LOFAR_L1_LDS_MAX=2).  Run in a child process with the setting applied."""
+    LOFAR_L1_LDS_MAX=2); "stage=b": LOFAR_BUCKET_STAGE=b (0: no scatter staged through LDS,
+    1: level 1 only, 2: level 2 only).  "1,100000,256" overflows the staged level 1's
+    round buffer (corner samples: 4 records each).  Run in a child process with the
+    setting applied."""
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    env = dict(os.environ, LOFAR_L1_LDS_MAX="2") if cfg == "global" else \
-        dict(os.environ, LOFAR_BUCKET_CFG=cfg)
+    if cfg == "global":
+        env = dict(os.environ, LOFAR_L1_LDS_MAX="2")
+    elif cfg.startswith("stage="):
+        env = dict(os.environ, LOFAR_BUCKET_STAGE=cfg[6:])
+    else:
+        env = dict(os.environ, LOFAR_BUCKET_CFG=cfg)
     r = subprocess.run([sys.executable, "-c", _BUCKET_SCRIPT, root], env=env, cwd=root,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "BUCKETS_OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32c", 1e-6)])
+def test_grid_and_degrid_tile_corner_samples(cuda, precision, tol):
+    """Samples at tile corners (4 entries per record: more than a staged level-2 round's
+    LDS buffer holds, the rest stored directly) grid to the oracle's cube; their
+    degridding (home sort) matches the oracle's."""
+    import torch
+    from lofar_amd import imager
+    nx, nw = 512, 2
+    d = corner_case(24, 30000, nx, nw)
+    st, want = O.wstack(nw, d["uu"], d["vv"], d["ww"], d["vis_real"], d["vis_img"], d["weight"],
+                        1 / nx, 1 / nw, 7, nx, nx)
+    assert st == O.OK
+    assert rel(H(imager.grid(*dev_inputs(d, cuda), nw, 7, nx, nx, precision=precision)), want) < tol
+    uu, vv, ww, _, _, wt = dev_inputs(d, cuda)
+    mr, mi = imager.degrid_cube(torch.from_numpy(want.copy()).to(cuda), uu, vv, ww, wt, 7)
+    st, model = O.degrid(want, d["uu"], d["vv"], d["ww"], d["weight"], 1 / nx, 1 / nw, 7)
+    assert st == O.OK and rel(H(mr) + 1j * H(mi), model) < 1e-12
 
 
 def test_grid_accumulate_and_empty(cuda):
