@@ -848,42 +848,66 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
     return level2<kOutFp32>(g, b, (const Rec32*)rec32, (Entry32*)entries, st);
 }
 
-// A plan's entries, with this call's visibilities: each entry's sample index (left in
-// its real part's slot by the sort) replaced by the sample's channel/polarisation-
-// summed weighted visibility, as k_l1_scatter_grid forms it.
-template <typename Ent>
+// A plan's entries, with this call's visibilities, in two passes: every sample's
+// channel/polarisation-summed weighted visibility as k_l1_scatter_grid forms it (in the
+// samples' order, coalesced; rounded to float for Entry32, as the entries hold it), then
+// each entry's sample index (left in its real part's slot by the sort) replaced by its
+// sample's sum, one 8- or 16-byte gather per entry.
+template <typename V2>
+__global__ __launch_bounds__(256) void k_plan_vis(long long n, long long F, long long P,
+                                                  const float* vr, const float* vi,
+                                                  const float* wt, V2* vis)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sr, si;
+    sample_vis(i, F, P, vr, vi, wt, sr, si);
+    V2 x;
+    x.x = (decltype(x.x))sr;
+    x.y = (decltype(x.y))si;
+    vis[i] = x;
+}
+
+template <typename Ent, typename V2>
 __global__ __launch_bounds__(256) void k_plan_fill(long long total, const Ent* pent, Ent* ent,
-                                                   long long F, long long P, const float* vr,
-                                                   const float* vi, const float* wt)
+                                                   const V2* vis)
 {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
     Ent x = pent[e];
-    double sr, si;
-    if constexpr (sizeof(Ent) == sizeof(Entry64)) {
-        sample_vis(__double_as_longlong(x.vr), F, P, vr, vi, wt, sr, si);
-        x.vr = sr;
-        x.vi = si;
-    } else {
-        sample_vis((long long)__float_as_uint(x.vr), F, P, vr, vi, wt, sr, si);
-        x.vr = (float)sr;
-        x.vi = (float)si;
-    }
+    long long i;
+    if constexpr (sizeof(Ent) == sizeof(Entry64))
+        i = __double_as_longlong(x.vr);
+    else
+        i = (long long)__float_as_uint(x.vr);
+    const V2 v = vis[i];
+    x.vr = v.x;
+    x.vi = v.y;
     ent[e] = x;
 }
 
-hipError_t plan_fill(int precision, long long total, const void* pent, void* ent, long long F,
-                     long long P, const float* vr, const float* vi, const float* wt,
-                     hipStream_t st)
+size_t plan_vis_bytes(int precision, long long n)
+{
+    return (size_t)n * (precision == LOFAR_FP64 ? sizeof(double2) : sizeof(float2));
+}
+
+hipError_t plan_fill(int precision, long long n, long long total, const void* pent, void* ent,
+                     long long F, long long P, const float* vr, const float* vi, const float* wt,
+                     void* vis, hipStream_t st)
 {
     if (total <= 0) return hipSuccess;
-    const dim3 grid(blocks_for(total, 256));
-    if (precision == LOFAR_FP64)
-        hipLaunchKernelGGL((k_plan_fill<Entry64>), grid, dim3(256), 0, st, total,
-                           (const Entry64*)pent, (Entry64*)ent, F, P, vr, vi, wt);
-    else
-        hipLaunchKernelGGL((k_plan_fill<Entry32>), grid, dim3(256), 0, st, total,
-                           (const Entry32*)pent, (Entry32*)ent, F, P, vr, vi, wt);
+    const dim3 gs(blocks_for(n, 256)), grid(blocks_for(total, 256));
+    if (precision == LOFAR_FP64) {
+        hipLaunchKernelGGL((k_plan_vis<double2>), gs, dim3(256), 0, st, n, F, P, vr, vi, wt,
+                           (double2*)vis);
+        hipLaunchKernelGGL((k_plan_fill<Entry64, double2>), grid, dim3(256), 0, st, total,
+                           (const Entry64*)pent, (Entry64*)ent, (const double2*)vis);
+    } else {
+        hipLaunchKernelGGL((k_plan_vis<float2>), gs, dim3(256), 0, st, n, F, P, vr, vi, wt,
+                           (float2*)vis);
+        hipLaunchKernelGGL((k_plan_fill<Entry32, float2>), grid, dim3(256), 0, st, total,
+                           (const Entry32*)pent, (Entry32*)ent, (const float2*)vis);
+    }
     return hipGetLastError();
 }
 

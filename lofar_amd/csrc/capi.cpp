@@ -704,14 +704,16 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     }
     if (plan) {
         const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
+        const size_t ebytes = align256((size_t)plan->gtotal * esz);
         char* eb = nullptr;
-        r = take(c, ar, c.ents, align256((size_t)plan->gtotal * esz) + 256, &eb);
+        r = take(c, ar, c.ents, ebytes + align256(plan_vis_bytes(precision, n)) + 256, &eb);
         if (r) return r;
         Buckets b{};
         b.map = plan->gmap;
         b.offsets = plan->goff;
         HIPCHK(stage(c, kStScatter, st, [&] {
-            return plan_fill(precision, plan->gtotal, plan->gent, eb, F, P, vr, vi, wt, st);
+            return plan_fill(precision, n, plan->gtotal, plan->gent, eb, F, P, vr, vi, wt,
+                             eb + ebytes, st);
         }));
         HIPCHK(stage(c, kStGrid, st, [&] {
             return grid_tiles(precision, g, b, eb, cube, accumulate, occ, sm_log, st);

@@ -232,10 +232,12 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
                             const double* uu, const double* vv, const double* ww, const float* vr,
                             const float* vi, const float* wt, const Geometry& g, const Buckets& b,
                             Entry64* rec, void* entries, hipStream_t st, bool index_vis = false);
-// total entries of index_vis sorted entries (pent) with this call's visibilities, into ent
-hipError_t plan_fill(int precision, long long total, const void* pent, void* ent, long long F,
-                     long long P, const float* vr, const float* vi, const float* wt,
-                     hipStream_t st);
+// total entries of index_vis sorted entries (pent) with this call's visibilities of the n
+// samples, into ent; vis: plan_vis_bytes(precision, n) bytes of scratch
+size_t plan_vis_bytes(int precision, long long n);
+hipError_t plan_fill(int precision, long long n, long long total, const void* pent, void* ent,
+                     long long F, long long P, const float* vr, const float* vi, const float* wt,
+                     void* vis, hipStream_t st);
 // wt1 (one polarisation): the samples' weights, stored in the entries' spare word
 hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, const double* ww,
                             const float* wt1, const Geometry& g, const Buckets& b, DEntry* rec,
