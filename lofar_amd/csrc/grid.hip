@@ -1044,12 +1044,23 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(Geometry g, TileMap tm, co
     if (beg == end) return;
     const T2* plane = (const T2*)cube + w * (long long)g.ny * g.nx;
     const int tid = threadIdx.x;
-    for (int c = tid; c < RS * RS; c += 256) {
+    // every load of the thread's part of the region is issued before the first LDS
+    // store (a load-store loop keeps one load in flight per thread)
+    constexpr int NLD = (RS * RS + 255) / 256;
+    T2 xs[NLD];
+#pragma unroll
+    for (int k = 0; k < NLD; k++) {
+        const int c = tid + k * 256;
         const int rr = c / RS, cc = c % RS;
         const int gv = v0 - H + rr, gu = u0 - H + cc;
-        T2 x{0, 0};
-        if (gv >= 0 && gv < g.ny && gu >= 0 && gu < g.nx) x = plane[(long long)gv * g.nx + gu];
-        region[c] = x;
+        xs[k] = T2{0, 0};
+        if (c < RS * RS && gv >= 0 && gv < g.ny && gu >= 0 && gu < g.nx)
+            xs[k] = plane[(long long)gv * g.nx + gu];
+    }
+#pragma unroll
+    for (int k = 0; k < NLD; k++) {
+        const int c = tid + k * 256;
+        if (c < RS * RS) region[c] = xs[k];
     }
     __syncthreads();
     const double norm = 0.5 / LOFAR_REF_PI;
