@@ -458,6 +458,67 @@ __global__ __launch_bounds__(256) void k_l1_scatter_home(long long n, const doub
     }
 }
 
+// Staged form of k_l1_scatter_home (see k_l1_scatter_staged): one record per valid
+// sample, in its home tile's super-tile.
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_l1_scatter_home_staged(long long n, const double* uu, const double* vv,
+                              const double* ww, const float* wt1, Geometry g, TileMap tm,
+                              long long chunk, const long long* off1, const unsigned* base1,
+                              DEntry* rec)
+{
+    constexpr int NT = kStageThreads;
+    constexpr int CAP = L1Stage<DEntry>::CAP;
+    constexpr int U = L1Stage<DEntry>::U;
+    static_assert(U * NT <= CAP, "one record per sample: a round always fits the buffer");
+    __shared__ DEntry stage[CAP];
+    __shared__ unsigned short skey[CAP];
+    __shared__ long long dst[kStageMaxBins];
+    __shared__ unsigned cnt[kStageMaxBins], pos[kStageMaxBins], wsum[NT / 64];
+    StageBins<NT> sb{dst, cnt, pos, wsum};
+    const int tid = threadIdx.x;
+    long long gnext = 0;
+    if (tid < tm.nk1) {
+        gnext = off1[tid] + base1[(long long)blockIdx.x * tm.nk1 + tid];
+        cnt[tid] = 0u;
+    }
+    __syncthreads();
+    const long long beg = (long long)blockIdx.x * chunk;
+    const long long end = min(n, beg + chunk);
+    for (long long r0 = beg; r0 < end; r0 += (long long)NT * U) {
+        DEntry e[U];
+        int key[U];  // -1: no record
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long i0 = r0 + (long long)u * NT + tid;
+            const long long i = min(i0, end - 1);
+            SampleWin s;
+            key[u] = -1;
+            if (i0 < end && sample_window(uu[i], vv[i], ww[i], g, s)) {
+                int cu, cv;
+                home_cell(s, g, cu, cv);
+                key[u] = (int)tm.key1(s.gw, cv / kTile, cu / kTile);
+                e[u].pu = s.pu;
+                e[u].pv = s.pv;
+                e[u].idx = i;
+                e[u].pad = wt1 ? (long long)__float_as_uint(wt1[i]) : 0;
+                atomicAdd(&cnt[key[u]], 1u);
+            }
+        }
+        __syncthreads();
+        const unsigned total = sb.open(tm.nk1, gnext);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (key[u] < 0) continue;
+            const unsigned p = atomicAdd(&pos[key[u]], 1u);
+            stage[p] = e[u];
+            skey[p] = (unsigned short)key[u];
+        }
+        __syncthreads();
+        for (unsigned j = tid; j < total; j += NT) rec[dst[skey[j]] + j] = stage[j];
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // level 2
 
@@ -920,6 +981,9 @@ hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, con
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_l1_scatter_home<true>, dim3((unsigned)b.nwg), dim3(256), 0, st, n,
                            uu, vv, ww, wt1, g, b.map, b.chunk, b.off1, b.base1, rec);
+    } else if (n > 0 && (b.stage & kStageL1)) {
+        hipLaunchKernelGGL(k_l1_scatter_home_staged, dim3((unsigned)b.nwg), dim3(kStageThreads),
+                           0, st, n, uu, vv, ww, wt1, g, b.map, b.chunk, b.off1, b.base1, rec);
     } else if (n > 0) {
         hipLaunchKernelGGL(k_l1_scatter_home<false>, dim3((unsigned)b.nwg), dim3(256),
                            (size_t)b.map.nk1 * sizeof(unsigned), st, n, uu, vv, ww, wt1, g,
