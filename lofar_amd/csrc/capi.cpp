@@ -711,6 +711,7 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
         Buckets b{};
         b.map = plan->gmap;
         b.offsets = plan->goff;
+        b.nent = plan->gtotal;
         HIPCHK(stage(c, kStScatter, st, [&] {
             return plan_fill(precision, n, plan->gtotal, plan->gent, eb, F, P, vr, vi, wt,
                              eb + ebytes, st);
@@ -748,6 +749,7 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
     Entry64* rec = (Entry64*)eb;
     void* ents = eb + rbytes;
     level2_layout(b, total, eb + rbytes + ebytes);
+    b.nent = total;
     HIPCHK(stage(c, kStScatter, st, [&] {
         return bucket_sort_grid(precision, n, F, P, uu, vv, ww, vr, vi, wt, g, b, rec, ents, st);
     }));

@@ -1235,16 +1235,21 @@ static int grid_cfg()
     return cfg;
 }
 
-// threads per gridding workgroup: LOFAR_GRID_NT (256, 512 or 1024), for measurement
-static int grid_nt()
+// threads per gridding workgroup: LOFAR_GRID_NT (256, 512 or 1024) overrides, for
+// measurement.  Otherwise 512, and 256 for FP32C tiles averaging fewer than 256 entries
+// (their per-tile phases, not the LDS atomics, set the time: 8192x8192x128 on 256M samples
+// grids ~7 ms faster per image; 590 entries a tile ran 2.61 -> 2.83 ms slower at 256, and the
+// FP64 gridder at ~134 entries a tile 2.02 -> 2.12 ms).
+static int grid_nt(int precision, const Buckets& b)
 {
-    static int nt = -1;
-    if (nt < 0) {
+    static const int forced = [] {
         const char* s = getenv("LOFAR_GRID_NT");
-        nt = s ? atoi(s) : 512;
-        if (nt != 256 && nt != 1024) nt = 512;
-    }
-    return nt;
+        const int v = s ? atoi(s) : 0;
+        return (v == 256 || v == 512 || v == 1024) ? v : 0;
+    }();
+    if (forced) return forced;
+    if (precision != LOFAR_FP64 && b.nent > 0 && b.nent < 256 * b.map.nbuckets()) return 256;
+    return 512;
 }
 
 template <int WIN, int NT>
@@ -1276,7 +1281,7 @@ static void launch_grid(int precision, const Geometry& g, const Buckets& b, cons
                         void* cube, int accumulate, unsigned long long* occ, int sm_log,
                         hipStream_t st)
 {
-    switch (grid_nt()) {
+    switch (grid_nt(precision, b)) {
         case 1024: return launch_grid_nt<WIN, 1024>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
         case 256: return launch_grid_nt<WIN, 256>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);
         default: return launch_grid_nt<WIN, 512>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);

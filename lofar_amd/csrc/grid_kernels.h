@@ -201,6 +201,7 @@ struct Buckets {
     unsigned* base1;       // [nwg*nk1] each workgroup's first slot in each super-tile
                            // ([nk1] global cursors when global_l1)
     long long* offsets;    // [nbuckets+1] entries of every (w, tile) bucket, bucket order
+    long long nent = 0;    // entries in all buckets, when the host knows it (else 0)
     long long* partials;   // scan partials
     long long* total;      // host-visible total entry count (status[0])
     long long* total_rec;  // total record count (device)
