@@ -952,6 +952,20 @@ size_t plan_vis_bytes(int precision, long long n)
     return (size_t)n * (precision == LOFAR_FP64 ? sizeof(double2) : sizeof(float2));
 }
 
+hipError_t plan_vis(int precision, long long n, long long F, long long P, const float* vr,
+                    const float* vi, const float* wt, void* vis, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 gs(blocks_for(n, 256));
+    if (precision == LOFAR_FP64)
+        hipLaunchKernelGGL((k_plan_vis<double2>), gs, dim3(256), 0, st, n, F, P, vr, vi, wt,
+                           (double2*)vis);
+    else
+        hipLaunchKernelGGL((k_plan_vis<float2>), gs, dim3(256), 0, st, n, F, P, vr, vi, wt,
+                           (float2*)vis);
+    return hipGetLastError();
+}
+
 hipError_t plan_fill(int precision, long long n, long long total, const void* pent, void* ent,
                      long long F, long long P, const float* vr, const float* vi, const float* wt,
                      void* vis, hipStream_t st)

@@ -703,15 +703,29 @@ int grid_impl(DeviceCtx& c, int precision, int accumulate, int nw, long n, long 
         return LOFAR_OK;
     }
     if (plan) {
+        Buckets b{};
+        b.map = plan->gmap;
+        b.offsets = plan->goff;
+        b.nent = plan->gtotal;
+        if (grid_gathers(precision, accumulate)) {
+            // the gridder gathers each entry's sample sum: no filled copy of the entries
+            char* vb = nullptr;
+            r = take(c, ar, c.ents, align256(plan_vis_bytes(precision, n)) + 256, &vb);
+            if (r) return r;
+            b.vis = vb;
+            HIPCHK(stage(c, kStScatter, st, [&] {
+                return plan_vis(precision, n, F, P, vr, vi, wt, vb, st);
+            }));
+            HIPCHK(stage(c, kStGrid, st, [&] {
+                return grid_tiles(precision, g, b, plan->gent, cube, accumulate, occ, sm_log, st);
+            }));
+            return LOFAR_OK;
+        }
         const size_t esz = precision == LOFAR_FP64 ? sizeof(Entry64) : sizeof(Entry32);
         const size_t ebytes = align256((size_t)plan->gtotal * esz);
         char* eb = nullptr;
         r = take(c, ar, c.ents, ebytes + align256(plan_vis_bytes(precision, n)) + 256, &eb);
         if (r) return r;
-        Buckets b{};
-        b.map = plan->gmap;
-        b.offsets = plan->goff;
-        b.nent = plan->gtotal;
         HIPCHK(stage(c, kStScatter, st, [&] {
             return plan_fill(precision, n, plan->gtotal, plan->gent, eb, F, P, vr, vi, wt,
                              eb + ebytes, st);

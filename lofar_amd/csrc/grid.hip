@@ -602,6 +602,19 @@ __device__ __forceinline__ float half_value(unsigned long long word, double inv)
         return (float)__longlong_as_double((long long)word);
 }
 
+// Entry e; with vis (a plan's entries, whose real part's slot holds the sample index),
+// its visibility is the sample's summed visibility in vis.
+__device__ __forceinline__ Entry32 entry_at(const Entry32* entries, const float2* vis, long long e)
+{
+    Entry32 x = entries[e];
+    if (vis) {
+        const float2 v = vis[__float_as_uint(x.vr)];
+        x.vr = v.x;
+        x.vi = v.y;
+    }
+    return x;
+}
+
 // A tile in two passes through one 32 KiB LDS tile (cleared on entry): real parts,
 // stored to the cube's real halves, then imaginary parts; S is the fixed-point
 // scale (FX).
@@ -609,13 +622,14 @@ template <int WIN, int NT, bool FX>
 __device__ __forceinline__ void grid_halves(unsigned long long* acc, const Geometry& g, long long w,
                                             int u0, int v0, int trows, int tcols, int sm_log,
                                             double norm, int S, const Entry32* entries,
-                                            long long beg, long long end, float2* cube)
+                                            const float2* vis, long long beg, long long end,
+                                            float2* cube)
 {
     const int tid = threadIdx.x;
     const double scale = ldexp(1.0, S), inv = ldexp(1.0, -S);
     float* out = (float*)cube;
     for (long long e = beg + tid; e < end; e += NT)
-        accumulate_half<WIN, 0, FX>(acc, g, u0, v0, norm, scale, entries[e]);
+        accumulate_half<WIN, 0, FX>(acc, g, u0, v0, norm, scale, entry_at(entries, vis, e));
     __syncthreads();
     for (int c = tid; c < kPlane; c += NT) {
         const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
@@ -625,7 +639,7 @@ __device__ __forceinline__ void grid_halves(unsigned long long* acc, const Geome
     for (int c = tid; c < kPlane / 2; c += NT) ((ulonglong2*)acc)[c] = ulonglong2{0ull, 0ull};
     __syncthreads();
     for (long long e = beg + tid; e < end; e += NT)
-        accumulate_half<WIN, 1, FX>(acc, g, u0, v0, norm, scale, entries[e]);
+        accumulate_half<WIN, 1, FX>(acc, g, u0, v0, norm, scale, entry_at(entries, vis, e));
     __syncthreads();
     for (int c = tid; c < kPlane; c += NT) {
         const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
@@ -635,8 +649,9 @@ __device__ __forceinline__ void grid_halves(unsigned long long* acc, const Geome
 
 template <int WIN, int NT>
 __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, const long long* offsets,
-                                                       const Entry32* entries, float2* cube,
-                                                       unsigned long long* occ, int sm_log)
+                                                       const Entry32* entries, const float2* vis,
+                                                       float2* cube, unsigned long long* occ,
+                                                       int sm_log)
 {
     __shared__ __align__(16) unsigned long long acc[kPlane];
     __shared__ unsigned long long bsum[kNBlk * kNBlk];
@@ -672,8 +687,8 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
     const long long n = end - beg;
     const int lane = tid & 63, wave = tid >> 6;
     Entry32 en0 = {0u, 0u, 0.0f, 0.0f}, en1 = {0u, 0u, 0.0f, 0.0f};
-    if (tid < n) en0 = entries[beg + tid];
-    if (NT + tid < n) en1 = entries[beg + NT + tid];
+    if (tid < n) en0 = entry_at(entries, vis, beg + tid);
+    if (NT + tid < n) en1 = entry_at(entries, vis, beg + NT + tid);
     for (int c = tid; c < kPlane / 2; c += NT)  // 16 bytes a store
         ((ulonglong2*)acc)[c] = ulonglong2{0ull, 0ull};  // also 0.0 as double
     for (int blk = tid; blk < kNBlk * kNBlk; blk += NT) bsum[blk] = 0ull;
@@ -686,7 +701,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
                 ((double)fabsf(en1.vr) + (double)fabsf(en1.vi));
     bool finite = isfinite(en0.vr) && isfinite(en0.vi) && isfinite(en1.vr) && isfinite(en1.vi);
     for (long long e = beg + 2 * NT + tid; e < end; e += NT) {
-        const Entry32 en = entries[e];
+        const Entry32 en = entry_at(entries, vis, e);
         finite = finite && isfinite(en.vr) && isfinite(en.vi);
         m = fmaxf(m, fmaxf(fabsf(en.vr), fabsf(en.vi)));
         sa += (double)fabsf(en.vr) + (double)fabsf(en.vi);
@@ -713,7 +728,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
     }
     if (!all_finite || (double)m > kHdrRatio * (sa - (double)m) / (double)(2 * n - 1)) {
         grid_halves<WIN, NT, false>(acc, g, w, u0, v0, trows, tcols, sm_log, norm, 0, entries,
-                                    beg, end, cube);
+                                    vis, beg, end, cube);
         return;
     }
     // the two-word sums also take tiles too large for the block counts and tiles
@@ -725,7 +740,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
         if (tid < n) add_block_bound<WIN>(bsum, g, u0, v0, rm, en0);
         if (NT + tid < n) add_block_bound<WIN>(bsum, g, u0, v0, rm, en1);
         for (long long e = beg + 2 * NT + tid; e < end; e += NT)
-            add_block_bound<WIN>(bsum, g, u0, v0, rm, entries[e]);
+            add_block_bound<WIN>(bsum, g, u0, v0, rm, entry_at(entries, vis, e));
         __syncthreads();
         for (int blk = tid; blk < kNBlk * kNBlk; blk += NT) {
             const unsigned long long b = bsum[blk];
@@ -745,7 +760,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
         int hb = 2;
         for (long long cnt = n; cnt > 1; cnt >>= 1) hb++;
         grid_halves<WIN, NT, true>(acc, g, w, u0, v0, trows, tcols, sm_log, norm,
-                                   kFixedBits - hb - ex, entries, beg, end, cube);
+                                   kFixedBits - hb - ex, entries, vis, beg, end, cube);
         return;
     }
     const int S = kPkHead - tile_ex;
@@ -753,7 +768,7 @@ __global__ __launch_bounds__(NT) void k_grid_tiles_pk(Geometry g, TileMap tm, co
     if (tid < n) accumulate_pk32<WIN>(acc, g, u0, v0, norm, scale, en0);
     if (NT + tid < n) accumulate_pk32<WIN>(acc, g, u0, v0, norm, scale, en1);
     for (long long e = beg + 2 * NT + tid; e < end; e += NT)
-        accumulate_pk32<WIN>(acc, g, u0, v0, norm, scale, entries[e]);
+        accumulate_pk32<WIN>(acc, g, u0, v0, norm, scale, entry_at(entries, vis, e));
     __syncthreads();
     for (int c = tid; c < kPlane; c += NT) {
         const FlushCell f = flush_cell(c, sm_log, g, w, u0, v0);
@@ -1267,7 +1282,8 @@ static void launch_grid_nt(int precision, const Geometry& g, const Buckets& b, c
                            b.offsets, entries, cube, accumulate, occ, sm_log);
     else if (!accumulate && cfg == 0)
         hipLaunchKernelGGL((k_grid_tiles_pk<WIN, NT>), grid, dim3(NT), 0, st, g, b.map, b.offsets,
-                           (const Entry32*)entries, (float2*)cube, occ, sm_log);
+                           (const Entry32*)entries, (const float2*)b.vis, (float2*)cube, occ,
+                           sm_log);
     else if (!accumulate && cfg == 2)
         hipLaunchKernelGGL((k_grid_tiles_fx<WIN, NT>), grid, dim3(NT), 0, st, g, b.map, b.offsets,
                            (const Entry32*)entries, (float2*)cube, occ, sm_log);
@@ -1348,6 +1364,11 @@ hipError_t degrid_direct(int precision, long long n, long long F, long long P, c
     return hipGetLastError();
 }
 
+bool grid_gathers(int precision, int accumulate)
+{
+    return precision != LOFAR_FP64 && !accumulate && grid_cfg() == 0;
+}
+
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
                       void* cube, int accumulate, unsigned long long* occ, int sm_log,
                       hipStream_t st)
@@ -1355,6 +1376,8 @@ hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const 
     // strip-major flushes need whole strips no wider than a tile
     if (sm_log >= 0 && (accumulate || (1 << sm_log) > kTile || g.nx % (1 << sm_log) != 0))
         return hipErrorInvalidValue;
+    // only the packed FP32C kernel gathers visibilities
+    if (b.vis && !grid_gathers(precision, accumulate)) return hipErrorInvalidValue;
     const int win = 2 * g.K + 1;
     if (win <= 7)
         launch_grid<7>(precision, g, b, entries, cube, accumulate, occ, sm_log, st);

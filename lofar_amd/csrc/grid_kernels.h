@@ -202,6 +202,8 @@ struct Buckets {
                            // ([nk1] global cursors when global_l1)
     long long* offsets;    // [nbuckets+1] entries of every (w, tile) bucket, bucket order
     long long nent = 0;    // entries in all buckets, when the host knows it (else 0)
+    const void* vis = nullptr;  // index entries: each sample's summed visibility (float2),
+                                // gathered by the gridder (grid_gathers)
     long long* partials;   // scan partials
     long long* total;      // host-visible total entry count (status[0])
     long long* total_rec;  // total record count (device)
@@ -236,6 +238,9 @@ hipError_t bucket_sort_grid(int precision, long long n, long long F, long long P
 // total entries of index_vis sorted entries (pent) with this call's visibilities of the n
 // samples, into ent; vis: plan_vis_bytes(precision, n) bytes of scratch
 size_t plan_vis_bytes(int precision, long long n);
+// each sample's summed visibility into vis (plan_fill's first pass alone)
+hipError_t plan_vis(int precision, long long n, long long F, long long P, const float* vr,
+                    const float* vi, const float* wt, void* vis, hipStream_t st);
 hipError_t plan_fill(int precision, long long n, long long total, const void* pent, void* ent,
                      long long F, long long P, const float* vr, const float* vi, const float* wt,
                      void* vis, hipStream_t st);
@@ -248,6 +253,8 @@ hipError_t bucket_sort_home(long long n, const double* uu, const double* vv, con
 // left unwritten (for a consumer that reads them as zeros).
 // sm_log >= 0 (without accumulate): the cube is written strip-major, each plane
 // [nx/L][ny][L] with L = 2^sm_log columns per strip.
+// whether grid_tiles takes index entries with b.vis (the packed FP32C kernel)
+bool grid_gathers(int precision, int accumulate);
 hipError_t grid_tiles(int precision, const Geometry& g, const Buckets& b, const void* entries,
                       void* cube, int accumulate, unsigned long long* occ, int sm_log,
                       hipStream_t st);
