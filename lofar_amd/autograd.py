@@ -89,18 +89,25 @@ class ImagingOperator:
                              self.weight, self.nw, self.support, self.resolution, self.wmin,
                              self.wmax, precision=self.precision, cube=self.cube)
 
-    def _image(self, vr, vi, **kw):
+    def _image(self, vr, vi, wt=None, **kw):
+        wt = self.unit if wt is None else wt
         pl = self.plan()
         if pl is not None:
-            return pl.image(vr, vi, self.unit, self.resolution, self.wmin, self.wmax,
+            return pl.image(vr, vi, wt, self.resolution, self.wmin, self.wmax,
                             cube=self.cube, **kw)
-        return imager.image(self.uu, self.vv, self.ww, vr, vi, self.unit, self.nw, self.support,
+        return imager.image(self.uu, self.vv, self.ww, vr, vi, wt, self.nw, self.support,
                             self.nx, self.ny, self.resolution, self.wmin, self.wmax,
                             precision=self.precision, cube=self.cube, **kw)
 
     def adjoint(self, vis_re, vis_im):
         """Dirty image (float64 re, im) of the weighted visibilities: A^H v."""
         torch = _torch()
+        if (self.precision != FP64 and vis_re.dtype == torch.float32 and
+                vis_im.dtype == torch.float32 and self.weight.dtype == torch.float32):
+            # float32 cotangents: the library's float product weight*vis is the correctly
+            # rounded exact product, the same value as rounding the double product below
+            return self._image(vis_re.contiguous(), vis_im.contiguous(),
+                               self.weight.contiguous())
         w = self.weight.to(torch.float64)
         gr = vis_re.to(torch.float64) * w
         gi = vis_im.to(torch.float64) * w
