@@ -29,7 +29,7 @@ from oracle import oracle as O
 
 def host_threads():
     """CPU threads this process may use: the scheduler affinity, capped by
-    OMP_NUM_THREADS when that is set (the GPU box presets it to its per-GPU share)."""
+    OMP_NUM_THREADS when that is set (a shared host may preset it to one GPU's share)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
