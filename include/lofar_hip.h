@@ -179,7 +179,8 @@ int lofar_degrid_dev(int precision, int num_w_planes, long num_points, long freq
  * for bit.  The degridding weights are those given to lofar_plan_create; the
  * imaging weights come with each call.  Windows wider than 17 cells keep copies of
  * uu, vv, ww and take the unsorted paths.  A plan belongs to the device current at
- * its creation (LOFAR_EARG elsewhere). */
+ * its creation (LOFAR_EARG elsewhere).  FP32C plans keep sample indices in 32 bits:
+ * 2^32 samples or more return LOFAR_ESIZE. */
 typedef struct lofar_plan_s* lofar_plan;
 int lofar_plan_create(int precision, int num_w_planes, long num_points, long freq_per_chan,
                       long polarizations, const double* uu, const double* vv, const double* ww,

@@ -1335,6 +1335,8 @@ int lofar_plan_create(int precision, int num_w_planes, long num_points, long fre
                             dw, w_support, grid_size_x, grid_size_y);
     if (r) return fail(r);
     if (num_points > 0 && (!uu || !vv || !ww || !weight)) return fail(LOFAR_EARG);
+    // an FP32C entry carries its sample's index in the 32 bits of its real part's slot
+    if (precision == LOFAR_FP32C && num_points > 0xffffffffL) return fail(LOFAR_ESIZE);
     CallGuard cg(stream);
     if (cg.status) return cg.status;
     DeviceCtx& c = *cg.c;

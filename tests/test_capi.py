@@ -41,3 +41,19 @@ def test_host_only_entry_points():
     assert L.lofar_error_string(_lib.ECOORD).decode().startswith("sample")
     assert L.lofar_last_error() == 0
     assert L.lofar_stage_name(3).decode() == "grid_tiles"
+
+
+def test_plan_create_rejects_sizes_before_touching_the_device():
+    """An FP32C plan keeps sample indices in 32 bits: 2^32 samples return LOFAR_ESIZE, and an
+    unsupported grid side too, from the argument checks alone (no device call, no plan)."""
+    import ctypes as C
+    L = _lib.lib()
+    fake = C.c_void_p(64)  # never dereferenced: the checks fail first
+    plan = C.c_void_p(12345)
+    args = (2, 1 << 32, 1, 1, fake, fake, fake, fake, 1.0 / 64, 0.5, 7, 64, 64, None,
+            C.byref(plan))
+    assert L.lofar_plan_create(_lib.FP32C, *args) == _lib.ESIZE and not plan.value
+    plan = C.c_void_p(12345)
+    assert L.lofar_plan_create(_lib.FP64, 2, 10, 1, 1, fake, fake, fake, fake, 1.0 / 64, 0.5,
+                               7, 64, 8193 * 2, None, C.byref(plan)) == _lib.ESIZE
+    assert not plan.value
