@@ -86,35 +86,40 @@ def test_plan_matches_oracle_and_outlives_the_callers_arrays(cuda, precision, to
     assert rel(H(mr) + 1j * H(mi), want_m) < (1e-10 if precision == "fp64" else 1e-5)
 
 
-def test_plan_channels_and_polarisations(cuda):
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_plan_channels_and_polarisations(cuda, precision):
     """Two channels and two polarisations: the planned image sums them with the call's
-    weights as lofar_image_dev does (bitwise), and the planned degridding writes every
-    (channel, polarisation) with the plan's weights as lofar_degrid_dev does."""
+    weights as lofar_image_dev does (bitwise; in FP32C through the per-sample sums the packed
+    gridder gathers), and the planned degridding writes every (channel, polarisation) with
+    the plan's weights as lofar_degrid_dev does."""
     import ctypes as C
     import torch
     from lofar_amd import imager
-    from lofar_amd._lib import FP64, check, lib
+    from lofar_amd._lib import FP32C, FP64, check, lib
     n, nx, nw, F, P = 20000, 128, 2, 2, 2
     d = random_case(75, n, nx, nx, nw, F=F, P=P, edge=True)
     uu, vv, ww, vr, vi, wt = dev_inputs(d, cuda)
-    plan = imager.Plan(uu, vv, ww, wt, nw, 7, nx, nx, precision="fp64", freq_per_chan=F,
+    plan = imager.Plan(uu, vv, ww, wt, nw, 7, nx, nx, precision=precision, freq_per_chan=F,
                        polarizations=P)
     args = (nw, 7, nx, nx, 1.0 / nx, -10.0, 10.0)
-    want = imager.image(uu, vv, ww, vr, vi, wt, *args, precision="fp64", freq_per_chan=F,
+    want = imager.image(uu, vv, ww, vr, vi, wt, *args, precision=precision, freq_per_chan=F,
                         polarizations=P)
     got = plan.image(vr, vi, wt, 1.0 / nx, -10.0, 10.0)
     assert _bits_equal(H(got[0]), H(want[0])) and _bits_equal(H(got[1]), H(want[1]))
+    fp64 = precision == "fp64"
     img = T(np.random.default_rng(76).standard_normal((nx, nx)), cuda)
     zero = torch.zeros_like(img)
-    cube = torch.empty((nw, nx, nx), dtype=torch.complex128, device=cuda)
-    mr = torch.empty(n * F * P, dtype=torch.float64, device=cuda)
+    cube = torch.empty((nw, nx, nx), dtype=torch.complex128 if fp64 else torch.complex64,
+                       device=cuda)
+    mr = torch.empty(n * F * P, dtype=torch.float64 if fp64 else torch.float32, device=cuda)
     mi = torch.empty_like(mr)
     s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     p = lambda t: C.c_void_p(t.data_ptr())
     check("lofar_degrid_dev", lib().lofar_degrid_dev(
-        FP64, nw, n, F, P, p(uu), p(vv), p(ww), p(wt), 1.0 / nx, 1.0 / nw, 7, nx, nx, 1.0 / nx,
-        -10.0, 10.0, p(img), p(zero), p(cube), p(mr), p(mi), s))
+        FP64 if fp64 else FP32C, nw, n, F, P, p(uu), p(vv), p(ww), p(wt), 1.0 / nx, 1.0 / nw, 7,
+        nx, nx, 1.0 / nx, -10.0, 10.0, p(img), p(zero), p(cube), p(mr), p(mi), s))
     gr, gi = plan.degrid(img, zero, 1.0 / nx, -10.0, 10.0)
+    assert gr.dtype == mr.dtype
     assert _bits_equal(H(gr), H(mr)) and _bits_equal(H(gi), H(mi))
 
 
