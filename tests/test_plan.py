@@ -162,3 +162,38 @@ def test_operator_with_and_without_plan_agree_bitwise(cuda, precision):
     ya, yb = a.adjoint(vr.double(), vi.double()), b.adjoint(vr.double(), vi.double())
     assert _bits_equal(H(ya[0]), H(yb[0])) and _bits_equal(H(ya[1]), H(yb[1]))
     assert a._plan is not None and b._plan is None
+
+
+@pytest.mark.parametrize("case", ["bright_outlier", "dense_block"])
+def test_plan_fp32c_fallback_tiles_match_unplanned(cuda, case):
+    """The packed FP32C gridder's fallbacks gather the plan's per-sample sums too: a tile with
+    one 4x4-cell block under 600 windows (two-word fixed-point halves: exact sums) gives the
+    unplanned image's bits; a tile whose brightest sample is 1e6 times the rest sums in double
+    with LDS atomics, whose order is not fixed, so it agrees to FP32C's tolerance."""
+    from lofar_amd import imager
+    n, nx, nw = 20000, 256, 2
+    d = random_case(81, n, nx, nx, nw, edge=False, nan=0.0)
+    rng = np.random.default_rng(82)
+    if case == "bright_outlier":
+        d["vis_real"] = d["vis_real"].copy()
+        d["vis_real"][123] = 1e6
+    else:
+        # 600 samples inside one 4x4-cell block of one plane
+        k = np.arange(600)
+        d["uu"] = d["uu"].copy()
+        d["vv"] = d["vv"].copy()
+        d["ww"] = d["ww"].copy()
+        d["uu"][k] = (100.0 + rng.uniform(0.0, 4.0, k.size)) / nx
+        d["vv"][k] = (60.0 + rng.uniform(0.0, 4.0, k.size)) / nx
+        d["ww"][k] = 0.25
+    uu, vv, ww, vr, vi, wt = dev_inputs(d, cuda)
+    plan = imager.Plan(uu, vv, ww, wt, nw, 7, nx, nx, precision="fp32c")
+    args = (nw, 7, nx, nx, 1.0 / nx, -10.0, 10.0)
+    want = imager.image(uu, vv, ww, vr, vi, wt, *args, precision="fp32c")
+    got = plan.image(vr, vi, wt, 1.0 / nx, -10.0, 10.0)
+    if case == "dense_block":
+        assert _bits_equal(H(got[0]), H(want[0])) and _bits_equal(H(got[1]), H(want[1]))
+    else:
+        e = max(rel_nan(H(got[0]), H(want[0])), rel_nan(H(got[1]), H(want[1])))
+        assert e < 1e-6, e
+    plan.close()
