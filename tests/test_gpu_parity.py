@@ -242,15 +242,17 @@ print("BUCKETS_OK")
 
 
 @pytest.mark.parametrize("cfg", ["1,256,256", "2,1000,300", "16,300,100000", "8,1000000,8192",
-                                 "1,100000,256", "global", "stage=0", "stage=1", "stage=2"])
+                                 "1,100000,256", "global", "stage=0", "stage=1", "stage=2",
+                                 "order=0", "order=4", "nt=256", "nt=1024"])
 def test_bucket_sort_configurations(cuda, cfg):
     """Other super-tile sides, level-1 chunk and level-2 block sizes (LOFAR_BUCKET_CFG, read once per
     process) give the same cube; "global": level 1 with global histograms and cursors (the
     path for more super-tiles than the LDS histograms hold, forced here with
     LOFAR_L1_LDS_MAX=2); "stage=b": LOFAR_BUCKET_STAGE=b (0: no scatter staged through LDS,
-    1: level 1 only, 2: level 2 only).  "1,100000,256" overflows the staged level 1's
-    round buffer (corner samples: 4 records each).  Run in a child process with the
-    setting applied."""
+    1: level 1 only, 2: level 2 only); "order=o": LOFAR_TILE_ORDER=o (launch order of the
+    per-tile kernels); "nt=t": LOFAR_GRID_NT=t (gridder workgroup size in both modes).
+    "1,100000,256" overflows the staged level 1's round buffer (corner samples: 4 records
+    each).  Run in a child process with the setting applied."""
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -258,6 +260,10 @@ def test_bucket_sort_configurations(cuda, cfg):
         env = dict(os.environ, LOFAR_L1_LDS_MAX="2")
     elif cfg.startswith("stage="):
         env = dict(os.environ, LOFAR_BUCKET_STAGE=cfg[6:])
+    elif cfg.startswith("order="):
+        env = dict(os.environ, LOFAR_TILE_ORDER=cfg[6:])
+    elif cfg.startswith("nt="):
+        env = dict(os.environ, LOFAR_GRID_NT=cfg[3:])
     else:
         env = dict(os.environ, LOFAR_BUCKET_CFG=cfg)
     r = subprocess.run([sys.executable, "-c", _BUCKET_SCRIPT, root], env=env, cwd=root,
