@@ -640,6 +640,27 @@ def test_degrid_matches_oracle_adjoint(cuda, precision, tol):
     assert rel(H(mr) + 1j * H(mi), want) < tol
 
 
+@pytest.mark.parametrize("support", [9, 13, 17])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-5)])
+def test_degrid_wide_tile_windows_match_oracle(cuda, precision, tol, support):
+    """The tile degridder's 9- and 17-cell window kernels (halos of 4 and 8 cells staged in
+    LDS, every region load issued before the LDS stores) against the oracle's interpolation,
+    samples reaching every grid edge."""
+    from lofar_amd import imager
+    nx, nw = 256, 4
+    d = random_case(16, 20000, nx, nx, nw, edge=True, nan=0)
+    res, wmin, wmax = 0.5 / nx, -30.0, 30.0  # every pixel inside l^2 + m^2 < 1
+    rng = np.random.default_rng(17)
+    img = rng.standard_normal((nx, nx)) + 1j * rng.standard_normal((nx, nx))
+    want = O.degrid_image(img, nw, d["uu"], d["vv"], d["ww"], d["weight"], support, res, wmin,
+                          wmax)
+    mr, mi = imager.degrid(T(img.real.copy(), cuda), T(img.imag.copy(), cuda), T(d["uu"], cuda),
+                           T(d["vv"], cuda), T(d["ww"], cuda), T(d["weight"], cuda), nw, support,
+                           res, wmin, wmax, precision=precision)
+    got = H(mr).astype(np.float64) + 1j * H(mi).astype(np.float64)
+    assert rel(got, want) < tol
+
+
 @pytest.mark.parametrize("nx,nw", [(256, 8), (8192, 1)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-5)])
 def test_degrid_stores_only_the_tiles_it_reads(cuda, precision, tol, nx, nw):
