@@ -501,7 +501,8 @@ def synth(n, nx, nw, seed):
 
 
 @pytest.mark.parametrize("n,nx,nw,support", [(30000, 256, 4, 7), (1 << 20, 1024, 8, 7),
-                                             (50000, 512, 16, 9)])
+                                             (50000, 512, 16, 9), (30000, 256, 4, 13),
+                                             (30000, 256, 4, 17)])
 def test_image_pipeline_matches_oracle(cuda, n, nx, nw, support):
     from lofar_amd import imager
     d = synth(n, nx, nw, seed=n + nx)
