@@ -237,6 +237,18 @@ for seed, n, nx, ny, nw in ((21, 40000, 300, 200, 3), (22, 3000, 1100, 64, 2),
     mr, mi = imager.degrid_cube(torch.from_numpy(want.copy()).to(dev), uu, vv, ww, wt, 7)
     st, model = O.degrid(want, d["uu"], d["vv"], d["ww"], d["weight"], 1 / nx, 1 / nw, 7)
     assert st == O.OK and rel(H(mr) + 1j * H(mi), model) < 1e-12
+    # a plan's sort (sample indices in the records) under the same setting: planned images
+    # carry the unplanned ones' bits
+    if nx == ny:
+        uu, vv, ww, vr, vi, wt = dev_inputs(d, dev)
+        for prec in ("fp64", "fp32c"):
+            plan = imager.Plan(uu, vv, ww, wt, nw, 7, nx, ny, precision=prec)
+            a = plan.image(vr, vi, wt, 0.5 / nx, -10.0, 10.0)
+            b = imager.image(uu, vv, ww, vr, vi, wt, nw, 7, nx, ny, 0.5 / nx, -10.0, 10.0,
+                             precision=prec)
+            for x, y in zip(a, b):
+                assert np.array_equal(H(x).view(np.uint8), H(y).view(np.uint8)), (seed, prec)
+            plan.close()
 print("BUCKETS_OK")
 """
 
