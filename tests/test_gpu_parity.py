@@ -218,7 +218,7 @@ def test_grid_accumulation_variants(cuda, grid_cfg):
 
 
 _BUCKET_SCRIPT = r"""
-import sys, numpy as np, torch
+import os, sys, numpy as np, torch
 sys.path.insert(0, sys.argv[1])
 from oracle import oracle as O
 from lofar_amd import imager
@@ -238,7 +238,9 @@ for seed, n, nx, ny, nw in ((21, 40000, 300, 200, 3), (22, 3000, 1100, 64, 2),
     st, model = O.degrid(want, d["uu"], d["vv"], d["ww"], d["weight"], 1 / nx, 1 / nw, 7)
     assert st == O.OK and rel(H(mr) + 1j * H(mi), model) < 1e-12
     # a plan's sort (sample indices in the records) under the same setting: planned images
-    # carry the unplanned ones' bits
+    # carry the unplanned ones' bits (to rounding where the gridder adds doubles in LDS,
+    # LOFAR_GRID_CFG=1, whose order is not fixed)
+    exact = os.environ.get("LOFAR_GRID_CFG", "0") != "1"
     if nx == ny:
         uu, vv, ww, vr, vi, wt = dev_inputs(d, dev)
         for prec in ("fp64", "fp32c"):
@@ -247,7 +249,10 @@ for seed, n, nx, ny, nw in ((21, 40000, 300, 200, 3), (22, 3000, 1100, 64, 2),
             b = imager.image(uu, vv, ww, vr, vi, wt, nw, 7, nx, ny, 0.5 / nx, -10.0, 10.0,
                              precision=prec)
             for x, y in zip(a, b):
-                assert np.array_equal(H(x).view(np.uint8), H(y).view(np.uint8)), (seed, prec)
+                if exact:
+                    assert np.array_equal(H(x).view(np.uint8), H(y).view(np.uint8)), (seed, prec)
+                else:
+                    assert rel(H(x), H(y)) < (1e-12 if prec == "fp64" else 1e-6), (seed, prec)
             plan.close()
 print("BUCKETS_OK")
 """
